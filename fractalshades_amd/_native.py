@@ -1,0 +1,209 @@
+"""ctypes binding of libfsgpu.so (declarations in include/fsgpu.h).
+
+The library is the only compute path of this package: if it is missing or
+cannot be loaded, every entry point raises ``FsgError`` -- there is no CPU
+fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libfsgpu.so")
+
+FSG_KIND_M2, FSG_KIND_MN, FSG_KIND_BS = 0, 1, 2
+FSG_PREC_FP64, FSG_PREC_FP32X = 0, 1
+FSG_INTERRUPTED = 1
+
+# entry points declared in include/fsgpu.h
+EXPORTS = (
+    "fsg_abi_version", "fsg_ctx_create", "fsg_ctx_destroy", "fsg_tile",
+    "fsg_tile_device", "fsg_work_bytes", "fsg_work_stats",
+    "fsg_ctx_read_table", "fsg_ctx_bla_shape", "fsg_last_error",
+)
+
+
+class FsgError(RuntimeError):
+    """Error raised by the native engine (or its absence)."""
+
+
+class RefDesc(ctypes.Structure):
+    """fsg_ref_desc (include/fsgpu.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("degree", ctypes.c_int32),
+        ("flavor", ctypes.c_int32), ("xr_mode", ctypes.c_int32),
+        ("ref_len", ctypes.c_int64), ("ref_div_iter", ctypes.c_int64),
+        ("ref_order", ctypes.c_int64),
+        ("Zn_path", ctypes.c_void_p),
+        ("n_xr", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("xr_index", ctypes.c_void_p), ("xr_mant", ctypes.c_void_p),
+        ("xr_exp", ctypes.c_void_p),
+        ("dx_mant", ctypes.c_double), ("dx_exp", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("drift_mant", ctypes.c_double * 2), ("drift_exp", ctypes.c_int32 * 2),
+        ("lin_mat", ctypes.c_double * 4),
+        ("kc_mant", ctypes.c_double), ("kc_exp", ctypes.c_int32), ("bla_on", ctypes.c_int32),
+        ("bla_eps", ctypes.c_double),
+        ("max_iter", ctypes.c_int64),
+        ("M_divergence", ctypes.c_double), ("epsilon_stationnary", ctypes.c_double),
+        ("calc_dzndc", ctypes.c_int32), ("interior_detect", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("pad2", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    """fsg_stats (include/fsgpu.h)."""
+    _fields_ = [
+        ("plain_steps", ctypes.c_uint64), ("bla_steps", ctypes.c_uint64),
+        ("bla_iters", ctypes.c_uint64), ("rebases", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=None):
+    """Load libfsgpu.so once; raise FsgError when it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise FsgError(
+                f"{p} is missing: build it with `python -m fractalshades_amd.build`"
+            )
+        try:
+            lib = ctypes.CDLL(p)
+        except OSError as exc:
+            raise FsgError(f"cannot load {p}: {exc}") from exc
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        sig = {
+            "fsg_abi_version": ([], ctypes.c_int),
+            "fsg_ctx_create": ([ctypes.POINTER(RefDesc), ctypes.POINTER(vp)], ctypes.c_int),
+            "fsg_ctx_destroy": ([vp], ctypes.c_int),
+            "fsg_tile": ([vp, vp, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
+            "fsg_tile_device": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "fsg_work_bytes": ([], ctypes.c_size_t),
+            "fsg_work_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
+            "fsg_ctx_read_table": ([vp, ctypes.c_int, vp, ctypes.c_size_t], ctypes.c_int),
+            "fsg_ctx_bla_shape": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i32)], ctypes.c_int),
+            "fsg_last_error": ([], ctypes.c_char_p),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        if lib.fsg_abi_version() != 1:
+            raise FsgError("libfsgpu.so ABI version mismatch")
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def check(rc):
+    """Map a negative return code to FsgError."""
+    if rc < 0:
+        msg = load().fsg_last_error()
+        raise FsgError(f"fsgpu error {rc}: {msg.decode() if msg else ''}")
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class GpuContext:
+    """A device-resident reference (fsg_ctx): orbit, derivative paths, BLA."""
+
+    def __init__(self, desc_fields, arrays):
+        lib = load()
+        # keep the input arrays alive until fsg_ctx_create has copied them
+        self._arrays = arrays
+        desc = RefDesc()
+        for k, v in desc_fields.items():
+            if k in ("drift_mant", "drift_exp", "lin_mat"):
+                arr = getattr(desc, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(desc, k, v)
+        desc.Zn_path = _ptr(arrays["Zn_path"])
+        desc.xr_index = _ptr(arrays["xr_index"])
+        desc.xr_mant = _ptr(arrays["xr_mant"])
+        desc.xr_exp = _ptr(arrays["xr_exp"])
+        desc.n_xr = int(arrays["xr_index"].size)
+        self.desc = desc
+        handle = ctypes.c_void_p()
+        check(lib.fsg_ctx_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self._h = handle
+        self.kind = desc.kind
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fsg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tile(self, c_pix, Z, U, stop_reason, stop_iter, interrupted=None):
+        """Run one tile in place; returns (rc, stats dict)."""
+        for name, arr in (("c_pix", c_pix), ("Z", Z), ("U", U),
+                          ("stop_reason", stop_reason), ("stop_iter", stop_iter)):
+            if not arr.flags.c_contiguous:
+                raise ValueError(f"{name} must be C-contiguous")
+        n = c_pix.shape[0]
+        if c_pix.dtype != np.complex128:
+            raise TypeError("c_pix must be complex128")
+        zdt = np.float64 if self.kind == FSG_KIND_BS else np.complex128
+        if Z.dtype != zdt or Z.shape[-1] != n:
+            raise TypeError("Z has the wrong dtype or shape")
+        if U.dtype != np.int32 or stop_iter.dtype != np.int32 or stop_reason.dtype != np.int8:
+            raise TypeError("U / stop_iter must be int32, stop_reason int8")
+        st = Stats()
+        intr = None
+        if interrupted is not None:
+            intr = interrupted.ctypes.data
+        rc = check(load().fsg_tile(
+            self._h, _ptr(c_pix), n, _ptr(Z), _ptr(U), _ptr(stop_reason), _ptr(stop_iter),
+            intr, ctypes.byref(st)))
+        return rc, st.as_dict()
+
+    def tile_device(self, c_pix_ptr, n, Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr):
+        """Enqueue one tile on device buffers (raw pointers); asynchronous."""
+        return check(load().fsg_tile_device(
+            self._h, c_pix_ptr, n, Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr))
+
+    def bla_shape(self):
+        bl = ctypes.c_int64()
+        st = ctypes.c_int32()
+        check(load().fsg_ctx_bla_shape(self._h, ctypes.byref(bl), ctypes.byref(st)))
+        return bl.value, st.value
+
+    def read_table(self, which, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        check(load().fsg_ctx_read_table(self._h, which, out.ctypes.data, out.nbytes))
+        return out
+
+
+def work_bytes():
+    return load().fsg_work_bytes()
+
+
+def work_stats(work_ptr):
+    st = Stats()
+    check(load().fsg_work_stats(work_ptr, ctypes.byref(st)))
+    return st.as_dict()

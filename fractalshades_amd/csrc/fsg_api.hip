@@ -1,0 +1,498 @@
+// fsg_api.hip -- C ABI of libfsgpu.so (declared in include/fsgpu.h).
+//
+// A context owns the device copy of one reference: orbit, Xrange register,
+// orbit-derivative paths (computed here on the host, serially, with the same
+// Xrange code as the kernels) and the BLA tree (built on the GPU).
+#include "fsgpu.h"
+#include "models.hpp"
+
+#include <mutex>
+#include <string>
+#include <vector>
+#include <cstring>
+#include <climits>
+
+using namespace fsg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) { g_err = msg; return code; }
+
+#define FSG_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(FSG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+constexpr int WORK_WORDS = 8;
+
+template <class T> T *dalloc(size_t n, hipError_t &err) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    err = hipMalloc(&p, n * sizeof(T));
+    return static_cast<T *>(p);
+}
+
+// ---- host-side orbit derivative paths (perturbation.py:2282-2516) ----
+
+cplx dfdz_rt(int kind, int deg, cplx z) {
+    if (kind == KIND_M2) return scale(2., z);
+    cplx tmp = z;
+    for (int q = 2; q < deg; q++) tmp = tmp * z;
+    return scale((double)deg, tmp);
+}
+xc dfdz_rt(int kind, int deg, xc z) {
+    if (kind == KIND_M2) return 2. * z;
+    xc tmp = z;
+    for (int q = 2; q < deg; q++) tmp = tmp * z;
+    return (double)deg * tmp;
+}
+
+template <class T> void jac_rt(int fl, T x, T y, T J[4]) {
+    switch (fl) {
+        case 1: bs_jac<1>(x, y, J); break;
+        case 2: bs_jac<2>(x, y, J); break;
+        case 3: bs_jac<3>(x, y, J); break;
+        case 4: bs_jac<4>(x, y, J); break;
+        default: bs_jac<5>(x, y, J); break;
+    }
+}
+
+struct HostRef {
+    const fsg_ref_desc *d;
+    int32_t L, div_iter, order;
+    cplx Z(int64_t i) const { return mk(d->Zn_path[2 * i], d->Zn_path[2 * i + 1]); }
+    int32_t find(int64_t i) const {
+        int32_t lo = 0, hi = d->n_xr;
+        while (lo < hi) {
+            int32_t mid = (lo + hi) >> 1;
+            if (d->xr_index[mid] < i) lo = mid + 1; else hi = mid;
+        }
+        return (lo < d->n_xr && d->xr_index[lo] == i) ? lo : -1;
+    }
+    // ref_path_get + ensure_xr (perturbation.py:2182-2193, 2519-2588)
+    xc Zxr(int64_t i) const {
+        int32_t j = d->n_xr ? find(i) : -1;
+        if (j >= 0) return mkc(d->xr_mant[2 * j], d->xr_mant[2 * j + 1], d->xr_exp[j]);
+        return to_xc(Z(i));
+    }
+    void ZxrBS(int64_t i, xf &X, xf &Y) const {
+        int32_t j = d->n_xr ? find(i) : -1;
+        if (j >= 0) {
+            X = mkf(d->xr_mant[2 * j], d->xr_exp[2 * j]);
+            Y = mkf(d->xr_mant[2 * j + 1], d->xr_exp[2 * j + 1]);
+        } else {
+            X = to_xf(Z(i).re); Y = to_xf(Z(i).im);
+        }
+    }
+};
+
+// dZndc: D_0 = 0, D_i = f'(Z_{i-1}) D_{i-1} + s, wrapped value at [0] for cycles
+void holo_dzndc(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e) {
+    const fsg_ref_desc *d = h.d;
+    const int64_t L = h.L, valid = std::min<int64_t>(h.L, h.div_iter);
+    m.assign(L, mk(0., 0.)); e.assign(L, 0);
+    const xf s = mkf(d->dx_mant, d->dx_exp);
+    if (!d->xr_mode) {
+        const double ss = to_std(s);
+        for (int64_t i = 1; i < valid; i++) m[i] = dfdz_rt(d->kind, d->degree, h.Z(i - 1)) * m[i - 1] + mk(ss, 0.);
+        int64_t i = valid - 1;
+        if (i == (int64_t)d->ref_order - 1) m[0] = dfdz_rt(d->kind, d->degree, h.Z(i)) * m[i] + mk(ss, 0.);
+    } else {
+        std::vector<xc> D(L, mkc(0., 0., 0));
+        for (int64_t i = 1; i < valid; i++) D[i] = dfdz_rt(d->kind, d->degree, h.Zxr(i - 1)) * D[i - 1] + s;
+        int64_t i = valid - 1;
+        if (i == (int64_t)d->ref_order - 1) D[0] = D[i] * dfdz_rt(d->kind, d->degree, h.Z(i)) + s;
+        for (int64_t k = 0; k < L; k++) { m[k] = mk(D[k].re, D[k].im); e[k] = D[k].e; }
+    }
+}
+
+// dZndz: [1] = 1, D_i = f'(Z_{i-1}) D_{i-1}, wrapped value stored at [L]
+void holo_dzndz(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e) {
+    const fsg_ref_desc *d = h.d;
+    const int64_t L = h.L, valid = std::min<int64_t>(h.L, h.div_iter);
+    m.assign(L + 1, mk(0., 0.)); e.assign(L + 1, 0);
+    if (!d->xr_mode) {
+        m[1] = mk(1., 0.);
+        for (int64_t i = 2; i < valid; i++) m[i] = dfdz_rt(d->kind, d->degree, h.Z(i - 1)) * m[i - 1];
+        int64_t i = valid - 1;
+        m[L] = dfdz_rt(d->kind, d->degree, h.Z(i)) * m[i];
+    } else {
+        std::vector<xc> D(L + 1, mkc(0., 0., 0));
+        D[1] = mkc(1., 0., 0);
+        for (int64_t i = 2; i < valid; i++) D[i] = dfdz_rt(d->kind, d->degree, h.Zxr(i - 1)) * D[i - 1];
+        int64_t i = valid - 1;
+        D[L] = dfdz_rt(d->kind, d->degree, h.Zxr(i)) * D[i];
+        for (int64_t k = 0; k <= L; k++) { m[k] = mk(D[k].re, D[k].im); e[k] = D[k].e; }
+    }
+}
+
+// Burning-ship Jacobian paths (numba_dZndc_path_BS + incr_deriv_ref_BS)
+void bs_paths(const HostRef &h, std::vector<double> P[4], std::vector<int32_t> E[4]) {
+    const fsg_ref_desc *d = h.d;
+    const int64_t L = h.L, valid = std::min<int64_t>(h.L, h.div_iter);
+    for (int q = 0; q < 4; q++) { P[q].assign(L, 0.); E[q].assign(L, 0); }
+    if (!d->calc_dzndc) return;
+    const xf s = mkf(d->dx_mant, d->dx_exp);
+    auto incr_std = [&](int64_t from, int64_t to) {
+        double J[4];
+        cplx R = h.Z(from);
+        jac_rt<double>(d->flavor, R.re, R.im, J);
+        double xa = P[0][from], xb = P[1][from], ya = P[2][from], yb = P[3][from];
+        const double ss = to_std(s);
+        P[0][to] = J[0] * xa + J[1] * ya + ss;
+        P[1][to] = J[0] * xb + J[1] * yb;
+        P[2][to] = J[2] * xa + J[3] * ya;
+        P[3][to] = J[2] * xb + J[3] * yb - ss;
+    };
+    if (!d->xr_mode) {
+        for (int64_t i = 1; i < valid; i++) incr_std(i - 1, i);
+        if (valid - 1 == (int64_t)d->ref_order - 1) incr_std(valid - 1, 0);
+        return;
+    }
+    std::vector<xf> X[4];
+    for (int q = 0; q < 4; q++) X[q].assign(L, mkf(0., 0));
+    auto incr_xr = [&](int64_t from, int64_t to) {
+        xf Rx, Ry, J[4];
+        h.ZxrBS(from, Rx, Ry);
+        jac_rt<xf>(d->flavor, Rx, Ry, J);
+        xf xa = X[0][from], xb = X[1][from], ya = X[2][from], yb = X[3][from];
+        X[0][to] = J[0] * xa + J[1] * ya + s;
+        X[1][to] = J[0] * xb + J[1] * yb;
+        X[2][to] = J[2] * xa + J[3] * ya;
+        X[3][to] = J[2] * xb + J[3] * yb - s;
+    };
+    for (int64_t i = 1; i < valid; i++) incr_xr(i - 1, i);
+    if (valid - 1 == (int64_t)d->ref_order - 1) incr_xr(valid - 1, 0);
+    for (int q = 0; q < 4; q++)
+        for (int64_t k = 0; k < L; k++) { P[q][k] = X[q][k].m; E[q][k] = X[q][k].e; }
+}
+
+int stages_of(int64_t L) {
+    int s = 0;
+    while ((1LL << s) < L) s++;
+    return s;
+}
+
+}  // namespace
+
+struct fsg_ctx {
+    fsg_ref_desc d;
+    int32_t L, div_iter, order, first_invalid;
+    int device;
+    // device tables
+    cplx *Zn = nullptr, *dc = nullptr, *dz = nullptr;
+    int32_t *dc_e = nullptr, *dz_e = nullptr;
+    double *bs[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t *bs_e[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t *xr_idx = nullptr, *xr_e = nullptr;
+    double *xr_m = nullptr;
+    double *M = nullptr, *r = nullptr;
+    int64_t bla_len = 0;
+    int32_t stages = 0;
+    // staging for fsg_tile
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    char *buf = nullptr;
+    size_t buf_bytes = 0;
+    unsigned long long *work = nullptr;
+    std::vector<void *> owned;
+
+    ~fsg_ctx() {
+        for (void *p : owned) if (p) (void)hipFree(p);
+        if (buf) (void)hipFree(buf);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class T> int upload(fsg_ctx *c, T *&dst, const T *src, size_t n) {
+    hipError_t err;
+    dst = dalloc<T>(n, err);
+    if (err != hipSuccess) return fail(FSG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(err));
+    c->owned.push_back(dst);
+    if (n) FSG_HIP(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int validate(const fsg_ref_desc *d) {
+    if (!d) return fail(FSG_E_INVALID, "null descriptor");
+    if (d->kind < FSG_KIND_M2 || d->kind > FSG_KIND_BS) return fail(FSG_E_INVALID, "unknown kind");
+    if (d->kind == FSG_KIND_MN && (d->degree < 2 || d->degree > 8))
+        return fail(FSG_E_UNSUPPORTED, "Mandelbrot_N degree must be in 2..8");
+    if (d->kind == FSG_KIND_BS && (d->flavor < 1 || d->flavor > 5))
+        return fail(FSG_E_INVALID, "burning-ship flavour must be in 1..5");
+    if (d->ref_len < 2 || d->ref_len > INT32_MAX / 4) return fail(FSG_E_INVALID, "ref_len out of range");
+    if (!d->Zn_path) return fail(FSG_E_INVALID, "null Zn_path");
+    if (d->n_xr < 0 || (d->n_xr > 0 && (!d->xr_index || !d->xr_mant || !d->xr_exp)))
+        return fail(FSG_E_INVALID, "bad Xrange register");
+    for (int32_t j = 1; j < d->n_xr; j++)
+        if (d->xr_index[j] <= d->xr_index[j - 1]) return fail(FSG_E_INVALID, "Xrange register not sorted");
+    if (d->max_iter < 1 || d->max_iter + d->ref_len >= (int64_t)INT32_MAX)
+        return fail(FSG_E_INVALID, "max_iter + ref_len must stay below 2^31");
+    if (d->ref_div_iter < 1 || d->ref_div_iter > (int64_t)INT32_MAX)
+        return fail(FSG_E_INVALID, "ref_div_iter out of range");
+    if (d->ref_order < 1) return fail(FSG_E_INVALID, "ref_order must be >= 1");
+    if (d->precision != FSG_PREC_FP64) return fail(FSG_E_UNSUPPORTED, "precision mode not available");
+    if (d->bla_on && stages_of(d->ref_len) <= 3)
+        return fail(FSG_E_UNSUPPORTED, "BLA needs a reference orbit longer than 8");
+    int64_t valid = std::min(d->ref_len, d->ref_div_iter);
+    if (d->kind != FSG_KIND_BS && d->calc_dzndc && valid < 2) return fail(FSG_E_INVALID, "orbit too short for dzndc");
+    if (d->kind != FSG_KIND_BS && d->interior_detect && valid < 3) return fail(FSG_E_INVALID, "orbit too short for dzndz");
+    if (d->kind == FSG_KIND_BS && d->calc_dzndc && valid < 2) return fail(FSG_E_INVALID, "orbit too short for hessian");
+    return 0;
+}
+
+int fill_args(fsg_ctx *c, KArgs &a) {
+    std::memset(&a, 0, sizeof(a));
+    Tables &t = a.t;
+    t.Zn = c->Zn; t.L = c->L; t.div_iter = c->div_iter; t.order = c->order; t.first_invalid = c->first_invalid;
+    t.dc = c->dc; t.dc_e = c->dc_e; t.dz = c->dz; t.dz_e = c->dz_e;
+    for (int q = 0; q < 4; q++) { t.bs[q] = c->bs[q]; t.bs_e[q] = c->bs_e[q]; }
+    t.n_xr = c->d.n_xr; t.xr_idx = c->xr_idx; t.xr_m = c->xr_m; t.xr_e = c->xr_e;
+    t.M = c->M; t.r = c->r; t.stages_bla = c->stages; t.bla_on = c->d.bla_on;
+    for (int q = 0; q < 4; q++) a.lin[q] = c->d.lin_mat[q];
+    a.dx_m = c->d.dx_mant; a.dx_e = c->d.dx_exp;
+    a.drift_m[0] = c->d.drift_mant[0]; a.drift_m[1] = c->d.drift_mant[1];
+    a.drift_e[0] = c->d.drift_exp[0]; a.drift_e[1] = c->d.drift_exp[1];
+    a.max_iter = (int32_t)c->d.max_iter;
+    a.calc_dc = c->d.calc_dzndc ? 1 : 0;
+    a.calc_dz = (c->d.kind != FSG_KIND_BS && c->d.interior_detect) ? 1 : 0;
+    a.M2 = c->d.M_divergence * c->d.M_divergence;
+    a.eps2 = c->d.epsilon_stationnary * c->d.epsilon_stationnary;
+    if (c->d.kind == FSG_KIND_BS) a.nz = 2 + (a.calc_dc ? 4 : 0);
+    else a.nz = 1 + a.calc_dc + a.calc_dz;
+    return 0;
+}
+
+LaunchCfg cfg_of(const fsg_ctx *c) {
+    LaunchCfg g;
+    g.kind = c->d.kind; g.degree = c->d.degree; g.flavor = c->d.flavor; g.xr = c->d.xr_mode ? 1 : 0;
+    return g;
+}
+
+size_t z_bytes(const fsg_ctx *c, int64_t n) {
+    int nz = (c->d.kind == FSG_KIND_BS) ? 2 + (c->d.calc_dzndc ? 4 : 0)
+                                        : 1 + (c->d.calc_dzndc ? 1 : 0) + (c->d.interior_detect ? 1 : 0);
+    return (size_t)nz * (size_t)n * (c->d.kind == FSG_KIND_BS ? 8 : 16);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8_t *sr, int32_t *si,
+           unsigned long long *work, hipStream_t stream, bool timed) {
+    KArgs a;
+    fill_args(c, a);
+    a.pix = pix; a.npts = n; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si; a.work = work;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, work, WORK_WORDS);
+    FSG_HIP(hipGetLastError());
+    if (n == 0) return 0;
+    if (timed) FSG_HIP(hipEventRecord(c->ev0, stream));
+    FSG_HIP(launch_tile(cfg_of(c), a, stream));
+    if (timed) FSG_HIP(hipEventRecord(c->ev1, stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_abi_version(void) { return FSG_ABI_VERSION; }
+
+const char *fsg_last_error(void) { return g_err.c_str(); }
+
+size_t fsg_work_bytes(void) { return WORK_WORDS * sizeof(unsigned long long); }
+
+int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
+    if (!out) return fail(FSG_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = validate(desc);
+    if (rc) return rc;
+    fsg_ctx *c = new (std::nothrow) fsg_ctx();
+    if (!c) return fail(FSG_E_NOMEM, "out of host memory");
+    c->d = *desc;
+    c->d.Zn_path = nullptr; c->d.xr_index = nullptr; c->d.xr_mant = nullptr; c->d.xr_exp = nullptr;
+    if (c->d.kind == FSG_KIND_M2) c->d.degree = 2;
+    if (c->d.kind == FSG_KIND_BS) c->d.interior_detect = 0;
+    c->L = (int32_t)desc->ref_len;
+    c->div_iter = (int32_t)desc->ref_div_iter;
+    c->order = desc->ref_order > (int64_t)INT32_MAX ? INT32_MAX : (int32_t)desc->ref_order;
+    c->first_invalid = std::min(c->L, std::min(c->div_iter, c->order));
+    auto cleanup = [&](int code) { delete c; return code; };
+
+    if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(FSG_E_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return cleanup(fail(FSG_E_HIP, "stream/event creation failed"));
+
+    const int64_t L = c->L;
+    if ((rc = upload(c, c->Zn, reinterpret_cast<const cplx *>(desc->Zn_path), (size_t)L))) return cleanup(rc);
+    const int32_t nx = desc->n_xr;
+    const int holo = desc->kind != FSG_KIND_BS;
+    if ((rc = upload(c, c->xr_idx, desc->xr_index, (size_t)nx))) return cleanup(rc);
+    if ((rc = upload(c, c->xr_m, desc->xr_mant, (size_t)2 * nx))) return cleanup(rc);
+    if ((rc = upload(c, c->xr_e, desc->xr_exp, (size_t)(holo ? nx : 2 * nx)))) return cleanup(rc);
+
+    HostRef h{&c->d, c->L, c->div_iter, c->order};
+    // the derivative recurrences read the host orbit and register
+    fsg_ref_desc dh = c->d;
+    dh.Zn_path = desc->Zn_path; dh.xr_index = desc->xr_index; dh.xr_mant = desc->xr_mant; dh.xr_exp = desc->xr_exp;
+    h.d = &dh;
+    if (holo) {
+        std::vector<cplx> m; std::vector<int32_t> e;
+        if (c->d.calc_dzndc) {
+            holo_dzndc(h, m, e);
+            if ((rc = upload(c, c->dc, m.data(), m.size()))) return cleanup(rc);
+            if ((rc = upload(c, c->dc_e, e.data(), e.size()))) return cleanup(rc);
+        }
+        if (c->d.interior_detect) {
+            holo_dzndz(h, m, e);
+            if ((rc = upload(c, c->dz, m.data(), m.size()))) return cleanup(rc);
+            if ((rc = upload(c, c->dz_e, e.data(), e.size()))) return cleanup(rc);
+        }
+    } else if (c->d.calc_dzndc) {
+        std::vector<double> P[4]; std::vector<int32_t> E[4];
+        bs_paths(h, P, E);
+        for (int q = 0; q < 4; q++) {
+            if ((rc = upload(c, c->bs[q], P[q].data(), P[q].size()))) return cleanup(rc);
+            if ((rc = upload(c, c->bs_e[q], E[q].data(), E[q].size()))) return cleanup(rc);
+        }
+    }
+
+    if (c->d.bla_on) {
+        c->stages = stages_of(L);
+        int64_t comp = L >> 3;
+        c->bla_len = 2 * comp;
+        int dim = holo ? 4 : 8;
+        hipError_t err;
+        c->M = dalloc<double>((size_t)(c->bla_len * dim), err);
+        if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc BLA"));
+        c->owned.push_back(c->M);
+        c->r = dalloc<double>((size_t)c->bla_len, err);
+        if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc BLA"));
+        c->owned.push_back(c->r);
+        if (hipMemsetAsync(c->M, 0, (size_t)(c->bla_len * dim) * sizeof(double), c->stream) != hipSuccess ||
+            hipMemsetAsync(c->r, 0, (size_t)c->bla_len * sizeof(double), c->stream) != hipSuccess)
+            return cleanup(fail(FSG_E_HIP, "hipMemsetAsync BLA"));
+        double kc = to_std(mkf(desc->kc_mant, desc->kc_exp));
+        hipError_t be = launch_bla_build(desc->kind, c->d.degree, desc->flavor, c->Zn, c->L, kc,
+                                         desc->bla_eps, c->M, c->r, c->stages, c->stream);
+        if (be != hipSuccess) return cleanup(fail(FSG_E_HIP, std::string("BLA build: ") + hipGetErrorString(be)));
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(FSG_E_HIP, "BLA build sync"));
+    }
+    hipError_t err;
+    c->work = dalloc<unsigned long long>(WORK_WORDS, err);
+    if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc work"));
+    c->owned.push_back(c->work);
+    *out = c;
+    return 0;
+}
+
+int fsg_ctx_destroy(fsg_ctx *ctx) {
+    if (!ctx) return 0;
+    { std::lock_guard<std::mutex> lk(ctx->mu); (void)hipStreamSynchronize(ctx->stream); }
+    delete ctx;
+    return 0;
+}
+
+int fsg_ctx_bla_shape(fsg_ctx *c, int64_t *bla_len, int32_t *stages) {
+    if (!c) return fail(FSG_E_INVALID, "null context");
+    if (bla_len) *bla_len = c->bla_len;
+    if (stages) *stages = c->stages;
+    return 0;
+}
+
+int fsg_ctx_read_table(fsg_ctx *c, int which, void *out, size_t bytes) {
+    if (!c || !out) return fail(FSG_E_INVALID, "null argument");
+    const void *src = nullptr;
+    size_t n = 0;
+    const size_t L = (size_t)c->L;
+    const int dim = c->d.kind == FSG_KIND_BS ? 8 : 4;
+    switch (which) {
+        case 0: src = c->M; n = (size_t)c->bla_len * dim * 8; break;
+        case 1: src = c->r; n = (size_t)c->bla_len * 8; break;
+        case 2: src = c->dc; n = L * 16; break;
+        case 3: src = c->dc_e; n = L * 4; break;
+        case 4: src = c->dz; n = (L + 1) * 16; break;
+        case 5: src = c->dz_e; n = (L + 1) * 4; break;
+        case 6: case 7: case 8: case 9: src = c->bs[which - 6]; n = L * 8; break;
+        case 10: case 11: case 12: case 13: src = c->bs_e[which - 10]; n = L * 4; break;
+        default: return fail(FSG_E_INVALID, "unknown table");
+    }
+    if (!src) return fail(FSG_E_INVALID, "table not computed for this context");
+    if (bytes < n) return fail(FSG_E_INVALID, "output buffer too small");
+    std::lock_guard<std::mutex> lk(c->mu);
+    FSG_HIP(hipMemcpy(out, src, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fsg_tile(fsg_ctx *c, const double *c_pix, int64_t n, double *Z, int32_t *U, int8_t *stop_reason,
+             int32_t *stop_iter, const volatile uint8_t *interrupted, fsg_stats *stats) {
+    if (!c) return fail(FSG_E_INVALID, "null context");
+    if (n < 0 || (n > 0 && (!c_pix || !Z || !U || !stop_reason || !stop_iter)))
+        return fail(FSG_E_INVALID, "bad tile buffers");
+    if (interrupted && *interrupted) return FSG_INTERRUPTED;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t pb = (size_t)n * 16, zb = z_bytes(c, n), ub = (size_t)n * 4, rb = (size_t)n;
+    const size_t off_z = align256(pb), off_u = off_z + align256(zb), off_i = off_u + align256(ub);
+    const size_t off_r = off_i + align256(ub), total = off_r + align256(rb);
+    if (total > c->buf_bytes) {
+        if (c->buf) { (void)hipFree(c->buf); c->buf = nullptr; c->buf_bytes = 0; }
+        if (hipMalloc(&c->buf, total) != hipSuccess) { c->buf = nullptr; return fail(FSG_E_NOMEM, "hipMalloc tile buffers"); }
+        c->buf_bytes = total;
+    }
+    char *b = c->buf;
+    if (n > 0) FSG_HIP(hipMemcpyAsync(b, c_pix, pb, hipMemcpyHostToDevice, c->stream));
+    int rc = launch(c, reinterpret_cast<double *>(b), n, reinterpret_cast<double *>(b + off_z),
+                    reinterpret_cast<int32_t *>(b + off_u), reinterpret_cast<int8_t *>(b + off_r),
+                    reinterpret_cast<int32_t *>(b + off_i), c->work, c->stream, true);
+    if (rc) return rc;
+    if (n > 0) {
+        FSG_HIP(hipMemcpyAsync(Z, b + off_z, zb, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(U, b + off_u, ub, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(stop_iter, b + off_i, ub, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(stop_reason, b + off_r, rb, hipMemcpyDeviceToHost, c->stream));
+    }
+    FSG_HIP(hipStreamSynchronize(c->stream));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        int r2 = fsg_work_stats(c->work, stats);
+        if (r2) return r2;
+        float ms = 0.f;
+        if (n > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) stats->kernel_ms = ms;
+    }
+    if (interrupted && *interrupted) return FSG_INTERRUPTED;
+    return 0;
+}
+
+int fsg_tile_device(fsg_ctx *c, const double *d_c_pix, int64_t n, double *d_Z, int32_t *d_U,
+                    int8_t *d_stop_reason, int32_t *d_stop_iter, void *d_work, void *stream) {
+    if (!c) return fail(FSG_E_INVALID, "null context");
+    if (!d_work) return fail(FSG_E_INVALID, "null work buffer");
+    if (n < 0 || (n > 0 && (!d_c_pix || !d_Z || !d_U || !d_stop_reason || !d_stop_iter)))
+        return fail(FSG_E_INVALID, "bad tile buffers");
+    return launch(c, d_c_pix, n, d_Z, d_U, d_stop_reason, d_stop_iter,
+                  static_cast<unsigned long long *>(d_work), static_cast<hipStream_t>(stream), false);
+}
+
+int fsg_work_stats(const void *d_work, fsg_stats *stats) {
+    if (!d_work || !stats) return fail(FSG_E_INVALID, "null argument");
+    unsigned long long w[WORK_WORDS];
+    FSG_HIP(hipMemcpy(w, d_work, sizeof(w), hipMemcpyDeviceToHost));
+    stats->plain_steps = w[1]; stats->bla_steps = w[2]; stats->bla_iters = w[3]; stats->rebases = w[4];
+    return 0;
+}
+
+}  // extern "C"

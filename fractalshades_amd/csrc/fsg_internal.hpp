@@ -1,0 +1,53 @@
+// fsg_internal.hpp -- structures shared by the kernels and the C-ABI layer.
+#pragma once
+#include "xr.hpp"
+
+namespace fsg {
+
+enum Kind { KIND_M2 = 0, KIND_MN = 1, KIND_BS = 2 };
+
+// Read-only per-reference data, resident in HBM for the life of a context.
+struct Tables {
+    const cplx *Zn;            // [L] reference orbit Z_n rounded to binary64
+    int32_t L, div_iter, order, first_invalid;
+    // holomorphic orbit derivatives: dZn/dc [L], dZn/dz [L+1]; in Xrange
+    // mode the exponents sit in the parallel *_e arrays
+    const cplx *dc; const int32_t *dc_e;
+    const cplx *dz; const int32_t *dz_e;
+    // Burning-ship Jacobian paths dX/da, dX/db, dY/da, dY/db [L]
+    const double *bs[4]; const int32_t *bs_e[4];
+    // sparse Xrange register of orbit points below 1e-300 (sorted indices)
+    int32_t n_xr; int32_t pad0;
+    const int32_t *xr_idx; const double *xr_m; const int32_t *xr_e;
+    // BLA tree: 4 doubles (A, B complex) or 8 doubles (2x2 A, 2x2 B) per entry
+    const double *M; const double *r;
+    int32_t stages_bla; int32_t bla_on;
+};
+
+// One launch of the tile kernel.
+struct KArgs {
+    Tables t;
+    double lin[4];             // rotation/skew matrix (lin_mat)
+    double dx_m; int32_t dx_e; int32_t pad1;
+    double drift_m[2]; int32_t drift_e[2];
+    int32_t max_iter; int32_t calc_dc; int32_t calc_dz; int32_t pad2;
+    double M2, eps2;
+    // pixels: complex128 screen offsets in units of dx ("c_pix")
+    const double *pix;
+    int64_t npts;
+    // outputs, reference layout: Z [n_Z, npts], U/stop_iter int32, stop_reason int8
+    double *Z; int32_t *U; int8_t *stop_reason; int32_t *stop_iter;
+    int32_t nz; int32_t pad3;
+    // workspace: [0] pixel cursor, [1..4] step counters (plain, bla, bla iters, rebase)
+    unsigned long long *work;
+};
+
+struct LaunchCfg { int kind, degree, flavor, xr; };
+
+hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
+hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
+                            double kc, double eps, double *M, double *r, int32_t stages,
+                            hipStream_t stream);
+int tile_kernel_blocks(const LaunchCfg &cfg);
+
+}  // namespace fsg

@@ -1,0 +1,326 @@
+"""PerturbationFractal: host side of the perturbation path.
+
+Mirrors the reference class (src/fractalshades/perturbation.py:22-977):
+``zoom`` (43-135), reference-orbit search and caching (get_FP_orbit 651-768,
+compute_FP_orbit 808-854, ref_point_matching 211-253), path data
+(get_path_data 317-375), ``ref_point_kc`` (166-194) and the per-calculation
+preparation ``get_cycle_indep_args`` (431-562).  The per-tile loop
+(``numba_cycle_call``, 414-428) runs on the GPU through libfsgpu.so.
+"""
+import logging
+
+import mpmath
+import numpy as np
+
+from . import settings
+from . import mpmath_loop as fsFP
+from ._native import GpuContext, FSG_KIND_M2, FSG_KIND_MN, FSG_KIND_BS, FSG_PREC_FP64
+from .core import Fractal
+from .utils import zoom_options
+from .xrange import mpf_to_xr, mpc_to_xr, xr_to_float
+
+logger = logging.getLogger(__name__)
+
+REF_ORDER_NONE = 1 << 62   # perturbation.py:455-456
+
+
+class CycleIndepArgs:
+    """What the per-tile call needs, computed once per calculation: the
+    engine descriptor (fields + arrays) and, created on first use, the
+    device-resident context built from it."""
+
+    def __init__(self, fields, arrays, interrupted):
+        self.fields = fields
+        self.arrays = arrays
+        self.interrupted = interrupted
+        self.last_stats = None
+        self._ctx = None
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self._ctx = GpuContext(self.fields, self.arrays)
+        return self._ctx
+
+    def close(self):
+        if self._ctx is not None:
+            self._ctx.close()
+            self._ctx = None
+
+
+class PerturbationFractal(Fractal):
+    """Escape-time fractal z_{n+1} = f(z_n) + c computed by perturbation
+    around an arbitrary-precision reference orbit."""
+
+    def __init__(self, directory):
+        super().__init__(directory)
+        self._FP_params = None
+        self._Zn_path = None
+        self.precision_mode = FSG_PREC_FP64
+
+    # ---- zoom ----------------------------------------------------------------
+    @zoom_options
+    def zoom(self, *, precision: int, x, y, dx, nx: int, xy_ratio: float,
+             theta_deg: float, projection="cartesian", has_skew: bool = False,
+             skew_00: float = 1., skew_01: float = 0., skew_10: float = 0.,
+             skew_11: float = 1.):
+        """Frame definition; precision in decimal digits (perturbation.py:43-135)."""
+        self.prec = mpmath.libmp.dps_to_prec(precision)
+        with mpmath.workprec(self.prec):
+            self.x = mpmath.mpf(x)
+            self.y = mpmath.mpf(y)
+            self.dx = dx = mpmath.mpf(dx)
+        if projection not in ("cartesian", None) and type(projection).__name__ != "Cartesian":
+            raise NotImplementedError("only the Cartesian projection is supported")
+        self._skew = None
+        if has_skew:
+            self._skew = np.array(((skew_00, skew_01), (skew_10, skew_11)), dtype=np.float64)
+        self.dx_std = float(dx)
+        self.dx_xr = mpf_to_xr(dx)
+        self.lin_mat = self.get_lin_mat()
+        # precision check (perturbation.py:123-135), Cartesian min_local_scale = 1
+        with mpmath.workdps(6):
+            pix = dx / nx
+            required_dps = int(-mpmath.log10(pix / nx) + 1)
+        if required_dps > precision:
+            raise ValueError(
+                "Precision is too low for min. pixel size and shall be increased "
+                f"to {required_dps} (current setting: {precision})."
+            )
+
+    @property
+    def xr_detect_activated(self):
+        """Extended range needed below dx = 1e-300 (perturbation.py:152-155)."""
+        return bool(self.dx < settings.xrange_zoom_level)
+
+    # ---- reference orbit ------------------------------------------------------
+    @property
+    def FP_params(self):
+        return self._FP_params
+
+    @property
+    def Zn_path(self):
+        return self._Zn_path
+
+    def ref_point_matching(self):
+        """Reuse the stored orbit when the precision, location, max_iter and
+        model match (perturbation.py:211-253)."""
+        fp = self._FP_params
+        if fp is None:
+            return False
+        with mpmath.workprec(self.prec):
+            drift = (self.x + 1j * self.y) - fp["ref_point"]
+            loc_ok = abs(drift / self.dx) ** 2 < 1.e6
+        return (self.prec <= fp["prec"] + 10 and loc_ok
+                and fp["max_iter"] >= self.max_iter
+                and fp["init_kwargs"] == self.init_kwargs
+                and fp.get("xr") is not None
+                and fp["xr_detect"] == self.xr_detect_activated)
+
+    def get_FP_orbit(self, c0=None, newton="cv", order=None, max_newton=None):
+        """Find the reference point (ball method + Newton) and compute its orbit
+        (perturbation.py:651-768)."""
+        if self.ref_point_matching():
+            return
+        if self.dx > settings.newton_zoom_level:
+            self.compute_critical_orbit(self.critical_pt)
+            return
+        with mpmath.workprec(self.prec):
+            self._get_FP_orbit(c0, newton, order, max_newton)
+
+    def _get_FP_orbit(self, c0, newton, order, max_newton):
+        if c0 is None:
+            c0 = self.x + 1j * self.y
+        if settings.no_newton or newton is None or newton == "None":
+            self.compute_FP_orbit(c0, None)
+            return
+        if order is None:
+            order = self.ball_method(c0, self.dx * 1.0)
+            if order is None:
+                logger.warning("Ball method failed - Default to image center")
+                self.compute_FP_orbit(c0, None)
+                return
+        prec = self.prec
+        eps_pixel = self.dx * (1. / self.nx)
+        try:
+            newton_cv, nucleus = self.find_nucleus(c0, order, eps_pixel, prec, max_newton)
+            no_div = True
+        except NotImplementedError:
+            newton_cv, nucleus = self.find_any_nucleus(c0, order, eps_pixel, prec, max_newton)
+            no_div = False
+        attempt = 1
+        max_attempt = 2
+        while not newton_cv and attempt <= max_attempt:
+            attempt += 1
+            prec = int(1.25 * prec)
+            try:
+                newton_cv, nucleus = self.find_nucleus(c0, order, eps_pixel, prec, max_newton)
+                no_div = True
+            except NotImplementedError:
+                newton_cv, nucleus = self.find_any_nucleus(c0, order, eps_pixel, prec, max_newton)
+                no_div = False
+            if no_div and not newton_cv and attempt == max_attempt:
+                newton_cv, nucleus = self.find_any_nucleus(c0, order, eps_pixel, prec, max_newton)
+        if not newton_cv:
+            logger.warning("Newton descent failed - Default to image center")
+            order = None
+            nucleus = c0
+        self.compute_FP_orbit(nucleus, order)
+
+    def compute_critical_orbit(self, crit):
+        """Shallow frames: a constant orbit at the critical point, order 100
+        (perturbation.py:770-805)."""
+        div_iter = 100
+        self._FP_params = {
+            "ref_point": mpmath.mpc(crit), "prec": self.prec, "order": 100,
+            "max_iter": self.max_iter, "FP_code": self.FP_code, "xr": {},
+            "div_iter": div_iter, "init_kwargs": self.init_kwargs,
+            "xr_detect": self.xr_detect_activated,
+        }
+        self._Zn_path = np.full([div_iter + 1], complex(crit), dtype=np.complex128)
+
+    def compute_FP_orbit(self, ref_point, order=None):
+        """Arbitrary-precision orbit, wrapped at the cycle order
+        (perturbation.py:808-854)."""
+        max_iter = self.max_iter
+        ref_orbit_len = max_iter + 1
+        if order is not None:
+            ref_orbit_len = min(order, ref_orbit_len)
+        div_iter, Zn_path, xr = self.FP_loop(ref_orbit_len, ref_point)
+        self._FP_params = {
+            "ref_point": mpmath.mpc(ref_point), "prec": self.prec, "order": order,
+            "max_iter": max_iter, "FP_code": self.FP_code, "ref_orbit_len": ref_orbit_len,
+            "xr": xr, "div_iter": div_iter, "init_kwargs": self.init_kwargs,
+            "xr_detect": self.xr_detect_activated,
+        }
+        self._Zn_path = Zn_path
+
+    def set_reference(self, ref_point, order, Zn_path, div_iter, xr):
+        """Install an orbit computed elsewhere (a cache or another process)."""
+        self._FP_params = {
+            "ref_point": mpmath.mpc(ref_point), "prec": self.prec, "order": order,
+            "max_iter": self.max_iter, "FP_code": self.FP_code, "xr": xr,
+            "div_iter": div_iter, "init_kwargs": self.init_kwargs,
+            "xr_detect": self.xr_detect_activated,
+        }
+        self._Zn_path = Zn_path
+
+    # ---- path data / kc -------------------------------------------------------
+    def ref_point_kc(self):
+        """Bound of |c - ref| over the frame corners, x 1.1 (perturbation.py:166-194)."""
+        with mpmath.workprec(self.prec):
+            w, h = 1., 1. / self.xy_ratio
+            mat = self.lin_mat
+            ref = self.FP_params["ref_point"]
+            c0 = self.x + 1j * self.y
+            shift = ref - c0
+            best = mpmath.mpf(0)
+            for sx, sy in ((0.5, 0.5), (-0.5, 0.5), (-0.5, -0.5), (0.5, -0.5)):
+                cx = mat[0, 0] * (sx * w) + mat[0, 1] * (sy * h)
+                cy = mat[1, 0] * (sx * w) + mat[1, 1] * (sy * h)
+                corner = mpmath.mpc(cx, cy) * self.dx
+                d = abs(shift - corner)
+                if d > best:
+                    best = d
+            return mpf_to_xr(best * 1.1)
+
+    def get_path_data(self):
+        """Orbit, Xrange register as sorted arrays, div_iter, order and the
+        centre drift (perturbation.py:317-375)."""
+        fp = self.FP_params
+        ref_order = fp["order"]
+        ref_div_iter = fp["div_iter"]
+        if ref_order is not None:
+            ref_div_iter = self.max_iter + 1
+        idx = np.array(sorted(fp["xr"].keys()), dtype=np.int32)
+        with mpmath.workprec(self.prec):
+            if self.holomorphic:
+                drift = mpc_to_xr((self.x + 1j * self.y) - fp["ref_point"])
+                mant = np.zeros(2 * len(idx), dtype=np.float64)
+                exp = np.zeros(len(idx), dtype=np.int32)
+                for j, i in enumerate(idx):
+                    m_re, m_im, e = fp["xr"][int(i)]
+                    mant[2 * j], mant[2 * j + 1], exp[j] = m_re, m_im, e
+                drift_m = (drift[0], drift[1])
+                drift_e = (drift[2], 0)
+            else:
+                dxr = mpf_to_xr(self.x - fp["ref_point"].real)
+                dyr = mpf_to_xr(self.y - fp["ref_point"].imag)
+                mant = np.zeros(2 * len(idx), dtype=np.float64)
+                exp = np.zeros(2 * len(idx), dtype=np.int32)
+                for j, i in enumerate(idx):
+                    (xm, xe), (ym, ye) = fp["xr"][int(i)]
+                    mant[2 * j], exp[2 * j] = xm, xe
+                    mant[2 * j + 1], exp[2 * j + 1] = ym, ye
+                drift_m = (dxr[0], dyr[0])
+                drift_e = (dxr[1], dyr[1])
+        return {
+            "Zn_path": np.ascontiguousarray(self.Zn_path, dtype=np.complex128),
+            "xr_index": idx, "xr_mant": mant, "xr_exp": exp,
+            "ref_div_iter": int(ref_div_iter),
+            "ref_order": REF_ORDER_NONE if ref_order is None else int(ref_order),
+            "drift_mant": drift_m, "drift_exp": drift_e,
+        }
+
+    # ---- engine descriptor ----------------------------------------------------
+    def get_ref_desc(self, spec):
+        """Descriptor fields + arrays of one calculation (the engine input and
+        the oracle input are the same)."""
+        pd = self.get_path_data()
+        kc_m, kc_e = self.ref_point_kc()
+        if xr_to_float(kc_m, kc_e) == 0. and not self.xr_detect_activated:
+            raise RuntimeError("Resolution is too low for this zoom depth.")
+        fields = {
+            "kind": self.fsg_kind, "degree": getattr(self, "exponent", 2),
+            "flavor": getattr(self, "flavor_int", 1),
+            "xr_mode": int(self.xr_detect_activated),
+            "ref_len": int(pd["Zn_path"].shape[0]),
+            "ref_div_iter": pd["ref_div_iter"], "ref_order": pd["ref_order"],
+            "dx_mant": self.dx_xr[0], "dx_exp": self.dx_xr[1],
+            "drift_mant": pd["drift_mant"], "drift_exp": pd["drift_exp"],
+            "lin_mat": tuple(float(v) for v in self.lin_mat.ravel()),
+            "kc_mant": kc_m, "kc_exp": kc_e,
+            "bla_on": int(spec["BLA_activated"]),
+            "bla_eps": float(spec["BLA_eps"] or 0.),
+            "max_iter": int(spec["max_iter"]),
+            "M_divergence": float(spec["M_divergence"]),
+            "epsilon_stationnary": float(spec.get("epsilon_stationnary", 0.) or 0.),
+            "calc_dzndc": int(spec["calc_dzndc"]),
+            "interior_detect": int(spec.get("calc_dzndz", False)),
+            "precision": int(spec.get("precision", self.precision_mode)),
+        }
+        arrays = {
+            "Zn_path": pd["Zn_path"].view(np.float64),
+            "xr_index": pd["xr_index"], "xr_mant": pd["xr_mant"], "xr_exp": pd["xr_exp"],
+        }
+        return fields, arrays
+
+    def get_cycle_indep_args(self, initialize, iterate):
+        """Reference orbit, derivative paths and BLA tree resident on the GPU
+        (perturbation.py:431-562)."""
+        spec = iterate
+        self.get_FP_orbit()
+        fields, arrays = self.get_ref_desc(spec)
+        self.kc = (fields["kc_mant"], fields["kc_exp"])
+        return CycleIndepArgs(fields, arrays, self._interrupted)
+
+    @staticmethod
+    def numba_cycle_call(cycle_dep_args, cycle_indep_args):
+        """One tile on the GPU; returns 0, or 1 when interrupted
+        (perturbation.py:414-428)."""
+        c_pix, Z, U, stop_reason, stop_iter = cycle_dep_args
+        args = cycle_indep_args
+        rc, stats = args.ctx.tile(c_pix, Z, U, stop_reason, stop_iter, args.interrupted)
+        args.last_stats = stats
+        return rc
+
+    def calc_raw(self, calc_name, jitter=False, supersampling=None):
+        args = self._calc_data[calc_name]["cycle_indep_args"]
+        res = super().calc_raw(calc_name, jitter, supersampling)
+        if res is not None:
+            res["stats"] = args.last_stats
+        return res
+
+    # ---- period / nucleus helpers (model wrappers fill these in) ---------------
+    def ball_method(self, c, px, kind=1, M_divergence=1.e5):
+        return self._ball_method(c, px, self.max_iter, M_divergence)

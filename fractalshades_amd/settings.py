@@ -1,0 +1,17 @@
+"""Module-level settings read by the perturbation path.
+
+Same names and defaults as the reference's settings (src/fractalshades/
+settings.py:7-40, 82).  The GPU engine replaces the thread pool, so
+``enable_multithreading`` only selects whether tiles are batched into one
+launch (True) or sent one by one (False).
+"""
+enable_multithreading: bool = True
+skip_calc: bool = False
+newton_zoom_level: float = 1.e-5
+std_zoom_level: float = 1.e-8
+xrange_zoom_level: float = 1.e-300
+no_newton: bool = False
+inspect_calc: bool = False
+chunk_size: int = 200
+BLA_compression: int = 3
+postproc_dtype: str = "float32"

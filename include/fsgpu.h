@@ -1,0 +1,174 @@
+/*
+ * fsgpu.h -- C ABI of libfsgpu.so, the MI355X (gfx950) perturbation engine.
+ *
+ * The library replaces the per-tile numba call of the reference,
+ *   PerturbationFractal.numba_cycle_call(cycle_dep_args, cycle_indep_args)
+ *   (src/fractalshades/perturbation.py:414-428),
+ * and the per-calculation preparation behind cycle_indep_args
+ *   (get_cycle_indep_args, perturbation.py:431-562).
+ * Paths are relative to the Fractalshades source tree.
+ *
+ * Conventions: every function returns 0 on success and a negative FSG_E_*
+ * code on failure (message from fsg_last_error()); USER_INTERRUPTED (1, as
+ * in perturbation.py:984) is returned when the interrupt flag was set.  No
+ * torch or HIP types appear in the signatures; `stream` arguments are
+ * hipStream_t passed as void*.  Host arrays are C-ordered numpy buffers.
+ * All entry points are thread-safe; calls on one context are serialised.
+ */
+#ifndef FSGPU_H
+#define FSGPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSG_ABI_VERSION 1
+
+/* model kinds */
+#define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
+#define FSG_KIND_MN 1   /* Perturbation_mandelbrot_N, models/mandelbrot_Mn.py:387-735 */
+#define FSG_KIND_BS 2   /* Perturbation_burning_ship, models/burning_ship.py:863-1130 */
+
+/* precision of the delta iteration */
+#define FSG_PREC_FP64 0   /* reference semantics: binary64, Xrange when xr_mode */
+#define FSG_PREC_FP32X 1  /* binary32 mantissa + int32 exponent (not the drop-in) */
+
+/* error codes */
+#define FSG_OK 0
+#define FSG_INTERRUPTED 1
+#define FSG_E_INVALID (-1)
+#define FSG_E_HIP (-2)
+#define FSG_E_NOMEM (-3)
+#define FSG_E_UNSUPPORTED (-4)
+
+/*
+ * Everything the reference gathers in cycle_indep_args
+ * (get_cycle_indep_args, perturbation.py:431-562, path data from
+ * get_path_data, perturbation.py:317-375) and the options that
+ * calc_std_div bakes into `iterate` (mandelbrot_M2.py:432-588,
+ * mandelbrot_Mn.py:462-625, burning_ship.py:971-1130).
+ *
+ * Xrange values are (mantissa, int32 exponent) pairs: value = m * 2^e.
+ */
+typedef struct fsg_ref_desc {
+    int32_t kind;            /* FSG_KIND_* */
+    int32_t degree;          /* exponent d for FSG_KIND_MN (2..8), 2 otherwise */
+    int32_t flavor;          /* burning-ship flavour 1..5 (burning_ship.py:63-76) */
+    int32_t xr_mode;         /* xr_detect_activated: dx < 1e-300 (perturbation.py:152-155) */
+    int64_t ref_len;         /* L = len(Zn_path) */
+    int64_t ref_div_iter;    /* first invalid orbit index (max_iter + 1 for cycles) */
+    int64_t ref_order;       /* cycle order; (1 << 62) when the orbit is not a cycle */
+    const double *Zn_path;   /* [2 L] interleaved complex128 */
+    int32_t n_xr;            /* size of the Xrange register (FP_params["xr"]) */
+    int32_t pad0;
+    const int32_t *xr_index; /* [n_xr] sorted orbit indices */
+    const double *xr_mant;   /* [2 n_xr]: complex mantissa (holomorphic) or (x, y) */
+    const int32_t *xr_exp;   /* [n_xr] (holomorphic) or [2 n_xr] (x, y) */
+    double dx_mant;          /* dx_xr = lin_scale_xr = derivative scale */
+    int32_t dx_exp;
+    int32_t pad1;
+    double drift_mant[2];    /* centre - reference point: complex (holomorphic, */
+    int32_t drift_exp[2];    /*   drift_exp[0]) or x / y reals (burning ship)   */
+    double lin_mat[4];       /* 2x2 rotation/skew matrix, row major (core.py:1470-1484) */
+    double kc_mant;          /* kc = bound of |c| over the frame (perturbation.py:166-194) */
+    int32_t kc_exp;
+    int32_t bla_on;          /* BLA_activated (mandelbrot_M2.py:521-524) */
+    double bla_eps;          /* BLA_eps */
+    int64_t max_iter;
+    double M_divergence;
+    double epsilon_stationnary;
+    int32_t calc_dzndc;      /* holomorphic calc_dzndc; burning ship calc_hessian */
+    int32_t interior_detect; /* holomorphic calc_dzndz (interior detection) */
+    int32_t precision;       /* FSG_PREC_* */
+    int32_t pad2;
+} fsg_ref_desc;
+
+/* plain / BLA step counters of the last call, for the FLOP accounting */
+typedef struct fsg_stats {
+    uint64_t plain_steps;    /* full perturbation iterations */
+    uint64_t bla_steps;      /* accepted BLA jumps */
+    uint64_t bla_iters;      /* iterations skipped by those jumps */
+    uint64_t rebases;        /* rebases (reference divergence + dynamic glitch) */
+    double kernel_ms;        /* device time of the tile kernel (HIP events) */
+} fsg_stats;
+
+typedef struct fsg_ctx fsg_ctx;
+
+/* Version of this ABI (FSG_ABI_VERSION). */
+int fsg_abi_version(void);
+
+/*
+ * Prepare a reference: uploads the orbit and Xrange register, computes the
+ * orbit derivative paths (numba_dZndc_path / numba_dZndz_path /
+ * numba_dZndc_path_BS, perturbation.py:2282-2516) and builds the BLA tree on
+ * the GPU (numba_make_BLA / numba_make_BLA_BS, perturbation.py:1819-2105).
+ * Replaces get_cycle_indep_args (perturbation.py:431-562).  Input arrays are
+ * copied; the caller may free them on return.
+ */
+int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out);
+
+/* Release a context and its device memory. */
+int fsg_ctx_destroy(fsg_ctx *ctx);
+
+/*
+ * One tile, host buffers in and out: the drop-in for numba_cycles_perturb /
+ * numba_cycles_perturb_BS (perturbation.py:988-1045, 1406-1451) behind
+ * numba_cycle_call (perturbation.py:414-428), cycle_dep_args =
+ * (c_pix, Z, U, stop_reason, stop_iter) of core.py:2044-2075:
+ *   c_pix       [2 n] complex128 pixel offsets in units of dx
+ *   Z           holomorphic: [n_Z, n] complex128 (zn, [dzndz], [dzndc]);
+ *               burning ship: [n_Z, n] float64 (xn, yn, [4 Jacobian])
+ *   U           [n] int32 (ref_cycle_iter)
+ *   stop_reason [n] int8
+ *   stop_iter   [n] int32
+ *   interrupted optional pointer to the reference's _interrupted flag
+ *               (core.py:1352); checked before and after the launch
+ *   stats       optional
+ * Synchronous.  Returns FSG_INTERRUPTED when *interrupted was set.
+ */
+int fsg_tile(fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z, int32_t *U,
+             int8_t *stop_reason, int32_t *stop_iter, const volatile uint8_t *interrupted,
+             fsg_stats *stats);
+
+/*
+ * Same computation on device buffers, enqueued on `stream` (hipStream_t) and
+ * returning immediately.  `work` is caller-owned device memory of at least
+ * fsg_work_bytes() bytes; it is cleared by the call and holds the pixel
+ * cursor and the step counters (read them with fsg_work_stats after the
+ * stream has completed).  No device allocation happens in this call.
+ * Replaces the same reference entry point as fsg_tile.
+ */
+int fsg_tile_device(fsg_ctx *ctx, const double *d_c_pix, int64_t n, double *d_Z,
+                    int32_t *d_U, int8_t *d_stop_reason, int32_t *d_stop_iter,
+                    void *d_work, void *stream);
+
+/* Size of the `work` buffer of fsg_tile_device. No reference counterpart. */
+size_t fsg_work_bytes(void);
+
+/* Copy the counters of a completed fsg_tile_device call to the host. No reference counterpart. */
+int fsg_work_stats(const void *d_work, fsg_stats *stats);
+
+/*
+ * Copy a derived table back to the host (for parity tests): which =
+ *   0: BLA M (M_bla, bla_len x 4 or 8 float64)   1: BLA radii (r_bla, bla_len)
+ *   2: dZndc path mantissas (2 L)   3: dZndc exponents (L, int32)
+ *   4: dZndz mantissas (2 (L + 1))  5: dZndz exponents (L + 1, int32)
+ *   6..9: burning-ship paths dXnda, dXndb, dYnda, dYndb (L float64)
+ *   10..13: their exponents (L int32)
+ * `bytes` is the size of `out`; returns FSG_E_INVALID if too small.  The
+ * tables are the reference's M_bla / r_bla / dZndc_path / dZndz_path
+ * (perturbation.py:1881, 2336, 2516).  fsg_ctx_bla_shape gives the BLA size.
+ */
+int fsg_ctx_read_table(fsg_ctx *ctx, int which, void *out, size_t bytes);
+int fsg_ctx_bla_shape(fsg_ctx *ctx, int64_t *bla_len, int32_t *stages_bla);
+
+/* Last error message of the calling thread. */
+const char *fsg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSGPU_H */

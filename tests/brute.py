@@ -1,0 +1,37 @@
+"""Brute-force ground truth: direct arbitrary-precision iteration of a pixel,
+without perturbation (mpmath)."""
+import mpmath
+
+
+def pixel_c(f, pix):
+    with mpmath.workprec(f.prec + 64):
+        return mpmath.mpc(f.x + mpmath.mpf(pix.real) * f.dx, f.y + mpmath.mpf(pix.imag) * f.dx)
+
+
+def holo_escape(f, pix, degree, max_iter, M, prec=600):
+    """(n, z_n, dz_n/dc * dx) at the first n with |z_n| > M, or at max_iter."""
+    c = pixel_c(f, pix)
+    M2 = M * M
+    with mpmath.workprec(prec):
+        z = mpmath.mpc(0)
+        dz = mpmath.mpc(0)
+        for n in range(1, max_iter + 1):
+            dz = degree * dz * (z ** (degree - 1) if degree > 2 else z) + 1
+            z = z ** degree + c if degree > 2 else z * z + c
+            if z.real ** 2 + z.imag ** 2 > M2:
+                break
+        return n, complex(z), complex(dz * f.dx)
+
+
+def bs_escape(f, pix, flavor, max_iter, M, prec=600):
+    from fractalshades_amd.mpmath_loop import bs_iterate
+    c = pixel_c(f, pix)
+    M2 = M * M
+    with mpmath.workprec(prec):
+        a, b = c.real, c.imag
+        x = y = mpmath.mpf(0)
+        for n in range(1, max_iter + 1):
+            x, y = bs_iterate(flavor, x, y, a, b)
+            if x * x + y * y > M2:
+                break
+        return n, complex(x, y)

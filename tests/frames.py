@@ -1,0 +1,54 @@
+"""Test frames: the reference test suite's zoom locations
+(tests/test_perturbation.py of Fractalshades) at reduced image sizes and
+iteration caps, plus helpers to run them through the oracle."""
+import numpy as np
+
+import fractalshades_amd as fsa
+
+# name: (model, zoom kwargs, calc kwargs) -- locations from the reference
+# tests (test_perturbation.py line numbers in the comments)
+FRAMES = {
+    # test_M2_E20, 66-69
+    "M2_E20": ("M2", dict(x="-1.74928893611435556407228", y="0.", dx="5.e-20", precision=40),
+               dict(max_iter=5000)),
+    # test_M2_int_E11, 138-141
+    "M2_E11": ("M2", dict(x="-1.74920463345912691e+00", y="-2.8684660237361114e-04",
+                          dx="5e-12", precision=30), dict(max_iter=5000)),
+    # test_glitch_divref, 308-312
+    "M2_divref": ("M2", dict(x="-1.36768994867991128", y="0.00949048853859240532",
+                             dx="2.477633848347765e-8", precision=40), dict(max_iter=2000)),
+    # test_glitch_divref5, 431-435
+    "M2_divref5": ("M2", dict(x="-1.8583883137588246496170054595",
+                              y="0.000062628913856889509589212598191",
+                              dx="4.688599337510725e-19", precision=50), dict(max_iter=5000)),
+    # test_deep_interior_detect, 584-588 (interior detection)
+    "M2_int32": ("M2", dict(x="-1.9072475829123560219806014933444037282313",
+                            y="7.54605431570315771754553123674989535657416e-16",
+                            dx="3.660343578223062e-32", precision=50),
+                 dict(max_iter=20000, interior_detect=True)),
+}
+
+
+def make(name, nx=48, model_kw=None, **calc_over):
+    model, zoom_kw, calc_kw = FRAMES[name]
+    if model == "M2":
+        f = fsa.Perturbation_mandelbrot("/tmp/fsa_test")
+    elif model == "MN":
+        f = fsa.Perturbation_mandelbrot_N("/tmp/fsa_test", **(model_kw or {}))
+    else:
+        f = fsa.Perturbation_burning_ship("/tmp/fsa_test", **(model_kw or {}))
+    f.zoom(nx=nx, xy_ratio=1.0, theta_deg=0., **zoom_kw)
+    kw = dict(calc_name="t", subset=None, M_divergence=1.e3, epsilon_stationnary=1.e-3,
+              BLA_eps=1.e-6)
+    if model == "BS":
+        kw.pop("epsilon_stationnary")
+    kw.update(calc_kw)
+    kw.update(calc_over)
+    f.calc_std_div(**kw)
+    return f
+
+
+def frame_inputs(f, calc_name="t"):
+    args = f._calc_data[calc_name]["cycle_indep_args"]
+    c_pix = np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny)))
+    return args, np.ascontiguousarray(c_pix)

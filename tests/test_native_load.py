@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every symbol of include/fsgpu.h
+(no compute call: this runs without a GPU)."""
+import os
+import re
+
+import fractalshades_amd._native as native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "fsgpu.h")).read()
+    return sorted(set(re.findall(r"^[\w ]+?\*?\s*(fsg_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "fsg_tile" in syms and "fsg_ctx_create" in syms
+    assert set(syms) == set(native.EXPORTS)
+
+
+def test_library_exports_all_declared():
+    lib = native.load()
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+    assert lib.fsg_abi_version() == 1
+    assert lib.fsg_work_bytes() >= 40
+
+
+def test_desc_layout_matches_header():
+    # offsets of a few fields of fsg_ref_desc, as laid out by the C compiler
+    d = native.RefDesc
+    assert d.Zn_path.offset == 40
+    assert d.dx_mant.offset == 80
+    assert d.lin_mat.offset == 120
+    assert d.precision.offset == 208
+    assert native.ctypes.sizeof(d) == 216
