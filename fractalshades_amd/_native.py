@@ -56,7 +56,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("plain_steps", ctypes.c_uint64), ("bla_steps", ctypes.c_uint64),
         ("bla_iters", ctypes.c_uint64), ("rebases", ctypes.c_uint64),
-        ("kernel_ms", ctypes.c_double),
+        ("bla_lookups", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -492,6 +492,7 @@ int fsg_work_stats(const void *d_work, fsg_stats *stats) {
     unsigned long long w[WORK_WORDS];
     FSG_HIP(hipMemcpy(w, d_work, sizeof(w), hipMemcpyDeviceToHost));
     stats->plain_steps = w[1]; stats->bla_steps = w[2]; stats->bla_iters = w[3]; stats->rebases = w[4];
+    stats->bla_lookups = w[5];
     return 0;
 }
 

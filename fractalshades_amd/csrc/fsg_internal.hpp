@@ -38,7 +38,8 @@ struct KArgs {
     // outputs, reference layout: Z [n_Z, npts], U/stop_iter int32, stop_reason int8
     double *Z; int32_t *U; int8_t *stop_reason; int32_t *stop_iter;
     int32_t nz; int32_t pad3;
-    // workspace: [0] pixel cursor, [1..4] step counters (plain, bla, bla iters, rebase)
+    // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
+    // rebase, bla lookups)
     unsigned long long *work;
 };
 

@@ -46,7 +46,7 @@ FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
     return cx + mkc(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
 }
 
-struct Counters { unsigned long long plain, bla, bla_it, reb; };
+struct Counters { unsigned long long plain, bla, bla_it, reb, tries; };
 
 FSG_HD cplx zc() { return mk(0., 0.); }
 FSG_HD xc zx() { return mkc(0., 0., 0); }
@@ -78,6 +78,7 @@ struct HoloStd {
         if (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.z.re, s.z.im, s.w, idx);
+            k.tries++;
             if (st != 0) {
                 const double *Mb = t.M + 4 * idx;
                 cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
@@ -158,6 +159,7 @@ struct HoloXr {
         if (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.zs.re, s.zs.im, s.w, idx);
+            k.tries++;
             if (st != 0) {
                 const double *Mb = t.M + 4 * idx;
                 cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
@@ -262,6 +264,7 @@ struct BsStd {
         if (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.x, s.y, s.w, idx);
+            k.tries++;
             if (st != 0) {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
@@ -341,6 +344,7 @@ struct BsXr {
         if (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.xs, s.ys, s.w, idx);
+            k.tries++;
             if (st != 0) {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
@@ -430,7 +434,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 template <class Model>
 __global__ void __launch_bounds__(256) k_tile(const KArgs a) {
     typename Model::State s;
-    Counters k = {0, 0, 0, 0};
+    Counters k = {0, 0, 0, 0, 0};
     int64_t idx = 0;
     bool need = true;
     for (;;) {
@@ -458,10 +462,11 @@ __global__ void __launch_bounds__(256) k_tile(const KArgs a) {
     }
     // all lanes have left the loop: one atomic per wave per counter
     unsigned long long c0 = wave_sum(k.plain), c1 = wave_sum(k.bla);
-    unsigned long long c2 = wave_sum(k.bla_it), c3 = wave_sum(k.reb);
+    unsigned long long c2 = wave_sum(k.bla_it), c3 = wave_sum(k.reb), c4 = wave_sum(k.tries);
     if (__lane_id() == 0) {
         atomicAdd(a.work + 1, c0); atomicAdd(a.work + 2, c1);
         atomicAdd(a.work + 3, c2); atomicAdd(a.work + 4, c3);
+        atomicAdd(a.work + 5, c4);
     }
 }
 

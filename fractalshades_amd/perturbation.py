@@ -153,6 +153,9 @@ class PerturbationFractal(Fractal):
         while not newton_cv and attempt <= max_attempt:
             attempt += 1
             prec = int(1.25 * prec)
+            # the reference raises the global mpmath precision here, so the
+            # orbit below is computed at the boosted precision too
+            self.prec = prec
             try:
                 newton_cv, nucleus = self.find_nucleus(c0, order, eps_pixel, prec, max_newton)
                 no_div = True

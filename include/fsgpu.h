@@ -92,6 +92,7 @@ typedef struct fsg_stats {
     uint64_t bla_steps;      /* accepted BLA jumps */
     uint64_t bla_iters;      /* iterations skipped by those jumps */
     uint64_t rebases;        /* rebases (reference divergence + dynamic glitch) */
+    uint64_t bla_lookups;    /* BLA table lookups (attempts, accepted or not) */
     double kernel_ms;        /* device time of the tile kernel (HIP events) */
 } fsg_stats;
 

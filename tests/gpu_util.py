@@ -1,0 +1,30 @@
+"""Helpers for GPU tests: run a frame on the engine with the reference's
+buffer layout."""
+import numpy as np
+
+
+def run_gpu(args, c_pix):
+    kind = args.fields["kind"]
+    n = c_pix.shape[0]
+    if kind == 2:
+        nz = 2 + (4 if args.fields["calc_dzndc"] else 0)
+        Z = np.zeros((nz, n), dtype=np.float64)
+    else:
+        nz = 1 + int(bool(args.fields["calc_dzndc"])) + int(bool(args.fields["interior_detect"]))
+        Z = np.zeros((nz, n), dtype=np.complex128)
+    U = np.zeros((1, n), dtype=np.int32)
+    sr = -np.ones((1, n), dtype=np.int8)
+    si = np.zeros((1, n), dtype=np.int32)
+    rc, stats = args.ctx.tile(np.ascontiguousarray(c_pix), Z, U, sr, si)
+    assert rc == 0
+    return (Z, U, sr, si), stats
+
+
+def assert_same(got, ref):
+    Zg, Ug, srg, sig = got
+    Zr, Ur, srr, sir = ref
+    np.testing.assert_array_equal(sig, sir)
+    np.testing.assert_array_equal(srg, srr)
+    np.testing.assert_array_equal(Ug, Ur)
+    # bit-exact (NaN-aware) on every output field
+    np.testing.assert_array_equal(Zg.view(np.uint64), Zr.view(np.uint64))

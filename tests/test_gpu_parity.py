@@ -1,0 +1,75 @@
+"""GPU engine vs the oracle, through the C ABI: bit-exact on every output
+(escape iteration, stop reason, reference position, z_n, dz_n/dc, dz_n/dz),
+for binary64 and Xrange deltas, BLA on and off, on the reference test
+suite's zoom locations and the seeded benchmark workloads."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import frames
+from gpu_util import run_gpu, assert_same
+from fractalshades_amd import configs
+from fractalshades_amd.perturbation import CycleIndepArgs
+
+pytestmark = pytest.mark.gpu
+
+
+def with_fields(args, **over):
+    return CycleIndepArgs(dict(args.fields, **over), args.arrays, None)
+
+
+@pytest.mark.parametrize("name", ["M2_E20", "M2_E11", "M2_divref", "M2_divref5", "M2_int32"])
+@pytest.mark.parametrize("xr", [0, 1])
+def test_m2_bitexact(name, xr):
+    f = frames.make(name, nx=64)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case {name} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+    assert stats["plain_steps"] > 0
+
+
+@pytest.mark.parametrize("bla", [0, 1])
+def test_m2_bla_switch_bitexact(bla):
+    f = frames.make("M2_E20", nx=64)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, bla_on=bla)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+    assert (stats["bla_steps"] > 0) == bool(bla)
+
+
+def test_bla_tree_and_paths_bitexact():
+    f = frames.make("M2_int32", nx=32)
+    args, _ = frames.frame_inputs(f)
+    o = orc.OracleCtx(args.fields, args.arrays)
+    M, r, stages = o.bla()
+    bl, st = args.ctx.bla_shape()
+    assert bl == r.size and st == stages
+    np.testing.assert_array_equal(args.ctx.read_table(0, M.shape, np.float64).view(np.uint64), M.view(np.uint64))
+    np.testing.assert_array_equal(args.ctx.read_table(1, r.shape, np.float64).view(np.uint64), r.view(np.uint64))
+    L = args.fields["ref_len"]
+    m, e = o.path(0)
+    np.testing.assert_array_equal(args.ctx.read_table(2, (2 * L,), np.float64), m)
+    m, e = o.path(1)
+    np.testing.assert_array_equal(args.ctx.read_table(4, (2 * (L + 1),), np.float64), m)
+
+
+@pytest.mark.parametrize("name", ["C1", "C2"])
+def test_workload_subsample_bitexact(name):
+    # full-size seeded benchmark frame on the GPU, checked on a strided
+    # subsample against the oracle
+    w = configs.WORKLOADS[name]
+    f = configs.build(w)
+    args = f._calc_data["bench"]["cycle_indep_args"]
+    c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
+    print(f"case {name} npts={c_pix.size}", flush=True)
+    got, stats = run_gpu(args, c_pix)
+    sel = np.arange(0, c_pix.size, 997)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix[sel])
+    sub = tuple(np.ascontiguousarray(a[..., sel]) for a in got)
+    assert_same(sub, ref)
+    assert np.all(got[2] >= 0)
