@@ -21,7 +21,9 @@ LIB = os.path.join(LIBDIR, "libfsgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["fsg_kernels.hip", "fsg_api.hip"]
+SOURCES = ["fsg_kernels.hip", "fsg_api.hip", "fsg_fp32x.hip"]
+# the binary32 form may fuse multiply-adds; the binary64 drop-in may not
+CONTRACT = {"fsg_fp32x.hip": "-ffp-contract=fast"}
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
@@ -50,7 +52,8 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmds.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        flags = [CONTRACT.get(src, f) if f == "-ffp-contract=off" else f for f in FLAGS]
+        cmds.append([HIPCC, *flags, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():

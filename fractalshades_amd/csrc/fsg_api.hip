@@ -197,6 +197,7 @@ struct fsg_ctx {
     double *xr_m = nullptr;
     double *M = nullptr, *r = nullptr;
     int64_t bla_len = 0;
+    TablesX tx = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int32_t stages = 0;
     // staging for fsg_tile
     std::mutex mu;
@@ -245,7 +246,8 @@ int validate(const fsg_ref_desc *d) {
     if (d->ref_div_iter < 1 || d->ref_div_iter > (int64_t)INT32_MAX)
         return fail(FSG_E_INVALID, "ref_div_iter out of range");
     if (d->ref_order < 1) return fail(FSG_E_INVALID, "ref_order must be >= 1");
-    if (d->precision != FSG_PREC_FP64) return fail(FSG_E_UNSUPPORTED, "precision mode not available");
+    if (d->precision != FSG_PREC_FP64 && d->precision != FSG_PREC_FP32X)
+        return fail(FSG_E_INVALID, "unknown precision mode");
     if (d->bla_on && stages_of(d->ref_len) <= 3)
         return fail(FSG_E_UNSUPPORTED, "BLA needs a reference orbit longer than 8");
     int64_t valid = std::min(d->ref_len, d->ref_div_iter);
@@ -263,6 +265,7 @@ int fill_args(fsg_ctx *c, KArgs &a) {
     for (int q = 0; q < 4; q++) { t.bs[q] = c->bs[q]; t.bs_e[q] = c->bs_e[q]; }
     t.n_xr = c->d.n_xr; t.xr_idx = c->xr_idx; t.xr_m = c->xr_m; t.xr_e = c->xr_e;
     t.M = c->M; t.r = c->r; t.stages_bla = c->stages; t.bla_on = c->d.bla_on;
+    a.tx = c->tx;
     for (int q = 0; q < 4; q++) a.lin[q] = c->d.lin_mat[q];
     a.dx_m = c->d.dx_mant; a.dx_e = c->d.dx_exp;
     a.drift_m[0] = c->d.drift_mant[0]; a.drift_m[1] = c->d.drift_mant[1];
@@ -280,6 +283,7 @@ int fill_args(fsg_ctx *c, KArgs &a) {
 LaunchCfg cfg_of(const fsg_ctx *c) {
     LaunchCfg g;
     g.kind = c->d.kind; g.degree = c->d.degree; g.flavor = c->d.flavor; g.xr = c->d.xr_mode ? 1 : 0;
+    g.precision = c->d.precision;
     return g;
 }
 
@@ -300,7 +304,8 @@ int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8
     FSG_HIP(hipGetLastError());
     if (n == 0) return 0;
     if (timed) FSG_HIP(hipEventRecord(c->ev0, stream));
-    FSG_HIP(launch_tile(cfg_of(c), a, stream));
+    if (c->d.precision == FSG_PREC_FP32X) FSG_HIP(launch_tile_f32(cfg_of(c), a, stream));
+    else FSG_HIP(launch_tile(cfg_of(c), a, stream));
     if (timed) FSG_HIP(hipEventRecord(c->ev1, stream));
     return 0;
 }
@@ -391,6 +396,27 @@ int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
                                          desc->bla_eps, c->M, c->r, c->stages, c->stream);
         if (be != hipSuccess) return cleanup(fail(FSG_E_HIP, std::string("BLA build: ") + hipGetErrorString(be)));
         if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(FSG_E_HIP, "BLA build sync"));
+    }
+    if (c->d.precision == FSG_PREC_FP32X) {
+        if (!f32_supported(cfg_of(c))) return cleanup(fail(FSG_E_UNSUPPORTED, "no FP32X kernel for this model"));
+        hipError_t xe;
+        void *p;
+        const size_t L1 = (size_t)L + 1;
+        p = dalloc<char>(L1 * XS_BYTES, xe); c->owned.push_back(p); c->tx.Zx = p;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        p = dalloc<char>(L1 * XS_BYTES, xe); c->owned.push_back(p); c->tx.DCx = c->dc ? p : nullptr;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        p = dalloc<char>((L1 + 1) * XS_BYTES, xe); c->owned.push_back(p); c->tx.DZx = c->dz ? p : nullptr;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        p = dalloc<char>((size_t)(c->bla_len + 1) * XS_BYTES, xe); c->owned.push_back(p); c->tx.Ax = p;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        p = dalloc<char>((size_t)(c->bla_len + 1) * XS_BYTES, xe); c->owned.push_back(p); c->tx.Bx = p;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        KArgs ka;
+        fill_args(c, ka);
+        hipError_t be = build_xs_tables(ka.t, c->bla_len, c->tx, c->stream);
+        if (be != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            return cleanup(fail(FSG_E_HIP, "FP32X table build failed"));
     }
     hipError_t err;
     c->work = dalloc<unsigned long long>(WORK_WORDS, err);

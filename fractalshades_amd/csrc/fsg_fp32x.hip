@@ -1,0 +1,266 @@
+// fsg_fp32x.hip -- the binary32-mantissa + int32-exponent form of the tile
+// kernel (FSG_PREC_FP32X).  Not the drop-in: it trades the reference's
+// binary64 / Xrange rounding for speed and reports its error against the
+// binary64 path.
+//
+// Every complex quantity of the delta loop (z, dz/dc, dz/dz, c, the orbit
+// and its derivative paths, the BLA coefficients) is a pair of floats whose
+// larger component is kept in [1, 2) plus an int32 exponent.  Normalising is
+// three instructions (v_frexp_exp_i32_f32, two v_ldexp_f32), with no
+// data-dependent branch, so the exponent range is unbounded for the cost of
+// binary32 arithmetic.  The escape test and the rebase tests that compare z
+// with the reference orbit run in binary64, on the binary64 orbit.
+//
+// The loop structure, BLA lookup and rebase rules are those of the binary64
+// kernels (numba_iterate, src/fractalshades/perturbation.py:1065-1400).
+#include "models.hpp"
+#include "tile_loop.hpp"
+
+namespace fsg {
+
+// ---------------------------------------------------------------------------
+// extended-exponent binary32 complex
+struct xs { float x, y; int32_t e; };
+
+constexpr int32_t XS_ZERO_E = -(1 << 28);   // exponent of 0: aligns to nothing
+
+__device__ __forceinline__ int32_t fexp(float v) { return __builtin_amdgcn_frexp_expf(v); }
+
+__device__ __forceinline__ xs xs_norm(float x, float y, int32_t e) {
+    float mx = fmaxf(fabsf(x), fabsf(y));
+    if (mx == 0.f) return xs{0.f, 0.f, XS_ZERO_E};
+    int32_t k = fexp(mx) - 1;                  // mx in [2^k, 2^(k+1))
+    return xs{ldexpf(x, -k), ldexpf(y, -k), e + k};
+}
+
+__device__ __forceinline__ xs operator+(xs a, xs b) {
+    int32_t E = max(a.e, b.e);
+    int32_t sa = max(a.e - E, -200), sb = max(b.e - E, -200);
+    return xs_norm(ldexpf(a.x, sa) + ldexpf(b.x, sb), ldexpf(a.y, sa) + ldexpf(b.y, sb), E);
+}
+__device__ __forceinline__ xs operator*(xs a, xs b) {
+    return xs_norm(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x, a.e + b.e);
+}
+__device__ __forceinline__ xs muls(double s, xs a) {
+    float f = (float)s;
+    return xs_norm(f * a.x, f * a.y, a.e);
+}
+__device__ __forceinline__ xs xs_zero() { return xs{0.f, 0.f, XS_ZERO_E}; }
+
+// binary64 value <-> xs
+__device__ __forceinline__ xs xs_from(double re, double im, int32_t e) {
+    double mx = fmax(fabs(re), fabs(im));
+    if (mx == 0.) return xs_zero();
+    int32_t k = __builtin_amdgcn_frexp_exp(mx) - 1;
+    return xs_norm((float)ldexp(re, -k), (float)ldexp(im, -k), e + k);
+}
+__device__ __forceinline__ cplx xs_to_d(xs a) {
+    int32_t e = max(a.e, -2000);
+    return mk(ldexp((double)a.x, e), ldexp((double)a.y, e));
+}
+// |a|^2 as (binary64 mantissa, exponent)
+__device__ __forceinline__ double xs_abs2_m(xs a) { return (double)a.x * a.x + (double)a.y * a.y; }
+__device__ __forceinline__ bool xs_abs2_le(xs a, xs b) {
+    // |a|^2 <= |b|^2 with exponents 2 a.e and 2 b.e
+    double ma = xs_abs2_m(a), mb = xs_abs2_m(b);
+    if (mb == 0.) return ma == 0.;
+    if (ma == 0.) return true;
+    int32_t d = 2 * (a.e - b.e);
+    d = max(min(d, 1100), -1100);
+    return ldexp(ma, d) <= mb;
+}
+
+// ---------------------------------------------------------------------------
+// per-reference tables in the xs format, built once per context
+__global__ void k_xs_from_path(const double *m, const int32_t *e, int32_t n, xs *out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = xs_from(m[2 * i], m[2 * i + 1], e ? e[i] : 0);
+}
+
+// orbit: binary64 values, with the Xrange register taking over below 1e-300
+__global__ void k_xs_orbit(const cplx *Zn, int32_t L, const int32_t *xr_idx, const double *xr_m,
+                           const int32_t *xr_e, int32_t n_xr, xs *out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    int32_t lo = 0, hi = n_xr;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (xr_idx[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    if (lo < n_xr && xr_idx[lo] == i) out[i] = xs_from(xr_m[2 * lo], xr_m[2 * lo + 1], xr_e[lo]);
+    else out[i] = xs_from(Zn[i].re, Zn[i].im, 0);
+}
+
+// BLA (A, B) pairs
+__global__ void k_xs_bla(const double *M, int64_t n, xs *A, xs *B) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    A[i] = xs_from(M[4 * i], M[4 * i + 1], 0);
+    B[i] = xs_from(M[4 * i + 2], M[4 * i + 3], 0);
+}
+
+hipError_t build_xs_tables(const Tables &t, int64_t bla_len, TablesX &x, hipStream_t stream) {
+    const int tb = 256;
+    hipLaunchKernelGGL(k_xs_orbit, dim3((t.L + tb - 1) / tb), dim3(tb), 0, stream, t.Zn, t.L,
+                       t.xr_idx, t.xr_m, t.xr_e, t.n_xr, (xs *)x.Zx);
+    if (t.dc) hipLaunchKernelGGL(k_xs_from_path, dim3((t.L + tb - 1) / tb), dim3(tb), 0, stream,
+                                 (const double *)t.dc, t.dc_e, t.L, (xs *)x.DCx);
+    if (t.dz) hipLaunchKernelGGL(k_xs_from_path, dim3((t.L + tb) / tb), dim3(tb), 0, stream,
+                                 (const double *)t.dz, t.dz_e, t.L + 1, (xs *)x.DZx);
+    if (t.bla_on && bla_len > 0)
+        hipLaunchKernelGGL(k_xs_bla, dim3((unsigned)((bla_len + tb - 1) / tb)), dim3(tb), 0, stream,
+                           t.M, bla_len, (xs *)x.Ax, (xs *)x.Bx);
+    return hipGetLastError();
+}
+
+// BLA stage selection of ref_BLA_get (perturbation.py:2116-2170), on |z|
+__device__ __forceinline__ int32_t bla_lookup_az(const Tables &t, double az, int32_t w, int32_t &idx) {
+    int32_t it = w >> 3;
+    int32_t tz = (it == 0) ? 31 : __builtin_ctz((uint32_t)it);
+    int32_t stg = min(3 + tz, t.stages_bla - 1);
+    int32_t invalid_step = t.first_invalid - w;
+    for (; stg >= 3; stg--) {
+        int32_t step = 1 << stg;
+        if (step >= invalid_step) continue;
+        int32_t k = 2 * it + (1 << (stg - 3)) - 1;
+        if (az < t.r[k]) { idx = k; return step; }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// holomorphic models
+template <int KIND, int DEG>
+struct HoloF32 {
+    struct State { xs z, d, h, c; int32_t w, n; bool dyn, nul; };
+
+    __device__ static void init(State &s, const KArgs &a, int64_t i) {
+        const double px = a.pix[2 * i], py = a.pix[2 * i + 1];
+        // c = dx * (lin_mat . pix) + drift, assembled in binary64 mantissas
+        const double x1 = a.lin[0] * px + a.lin[1] * py;
+        const double y1 = a.lin[2] * px + a.lin[3] * py;
+        xs cp = xs_from(a.dx_m * x1, a.dx_m * y1, a.dx_e);
+        xs dr = xs_from(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
+        s.c = cp + dr;
+        s.z = xs_zero(); s.d = xs_zero(); s.h = xs_zero();
+        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
+    }
+
+    __device__ static int step(State &s, const KArgs &a, Counters &k) {
+        const Tables &t = a.t;
+        const xs *Zx = (const xs *)a.tx.Zx, *DC = (const xs *)a.tx.DCx, *DZ = (const xs *)a.tx.DZx;
+        if (t.bla_on && (s.w & 7) == 0) {
+            const double az = ldexp((double)hypotf(s.z.x, s.z.y), max(s.z.e, -2000));
+            int32_t idx;
+            const int32_t st = bla_lookup_az(t, az, s.w, idx);
+            k.tries++;
+            if (st != 0) {
+                const xs A = ((const xs *)a.tx.Ax)[idx], B = ((const xs *)a.tx.Bx)[idx];
+                s.n += st;
+                s.w += st;
+                s.z = A * s.z + B * s.c;
+                if (a.calc_dc) s.d = A * s.d;
+                if (a.calc_dz) s.h = A * s.h;
+                k.bla++; k.bla_it += st;
+                return -1;
+            }
+        }
+        s.n += 1; k.plain++;
+        const xs Zr = Zx[s.w];
+        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xs_zero() : DC[s.w]);
+        if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ[0] : DZ[s.w]);
+        s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+        s.w += 1;
+        if (s.w >= t.order) s.w = 0;
+        if (s.n >= a.max_iter) return 0;
+        if (a.calc_dz) {
+            const xs q = s.h + (s.nul ? DZ[0] : (s.n == t.order ? DZ[t.L] : DZ[s.w]));
+            if (ldexp(xs_abs2_m(q), 2 * max(q.e, -2000)) < a.eps2) return 2;
+        }
+        // escape and rebase tests in binary64
+        const cplx zd = xs_to_d(s.z);
+        const cplx Zn1 = t.Zn[s.w];
+        const cplx ZZ = zd + Zn1;
+        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
+        if (s.w >= t.div_iter - 1) {
+            s.z = xs_from(ZZ.re, ZZ.im, 0);
+            if (a.calc_dc) s.d = s.d + DC[s.w];
+            if (a.calc_dz) {
+                if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
+                s.nul = true;
+            }
+            s.w = 0;
+            k.reb++;
+            return -1;
+        }
+        s.dyn = (fabs(ZZ.re) <= fabs(zd.re)) && (fabs(ZZ.im) <= fabs(zd.im));
+        if (s.dyn) {
+            const xs ZZx = s.z + Zx[s.w];
+            if (xs_abs2_le(ZZx, s.z)) {
+                s.z = ZZx;
+                if (a.calc_dc) s.d = s.d + DC[s.w];
+                if (a.calc_dz) {
+                    if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
+                    s.nul = true;
+                }
+                s.w = 0;
+                k.reb++;
+            }
+        }
+        return -1;
+    }
+
+    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
+        const int64_t n = a.npts;
+        const cplx z = xs_to_d(s.z) + a.t.Zn[s.w];
+        a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
+        int f = 1;
+        if (a.calc_dz) {
+            const cplx h = xs_to_d(s.h);
+            a.Z[2 * (n + i)] = h.re; a.Z[2 * (n + i) + 1] = h.im;
+            f = 2;
+        }
+        if (a.calc_dc) {
+            const cplx d = xs_to_d(s.d + ((const xs *)a.tx.DCx)[s.w]);
+            a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
+        }
+        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
+    }
+};
+
+typedef void (*tile_fn)(const KArgs);
+template <class Model> static tile_fn fn_of() { return k_tile<Model>; }
+
+static tile_fn select_f32(const LaunchCfg &c) {
+    if (c.kind == KIND_M2) return fn_of<HoloF32<KIND_M2, 2>>();
+    if (c.kind == KIND_MN) {
+        switch (c.degree) {
+#define FSG_MN(D) case D: return fn_of<HoloF32<KIND_MN, D>>();
+            FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
+#undef FSG_MN
+            default: return nullptr;
+        }
+    }
+    return nullptr;
+}
+
+hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
+    tile_fn f = select_f32(cfg);
+    if (!f) return hipErrorInvalidValue;
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return hipErrorInvalidValue;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorInvalidValue;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), 256, 0) != hipSuccess)
+        return hipErrorInvalidValue;
+    int blocks = ncu * (per_cu < 1 ? 1 : per_cu);
+    int64_t need = (a.npts + 255) / 256;
+    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
+    hipLaunchKernelGGL(f, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+bool f32_supported(const LaunchCfg &cfg) { return select_f32(cfg) != nullptr; }
+
+}  // namespace fsg

@@ -24,9 +24,19 @@ struct Tables {
     int32_t stages_bla; int32_t bla_on;
 };
 
+// The same tables in the binary32-mantissa + int32-exponent format of the
+// FSG_PREC_FP32X kernels (12-byte {float re, float im, int32 exp} records).
+struct TablesX {
+    const void *Zx;            // [L] orbit
+    const void *DCx;           // [L] dZn/dc
+    const void *DZx;           // [L+1] dZn/dz
+    const void *Ax, *Bx;       // [bla_len] BLA coefficients
+};
+
 // One launch of the tile kernel.
 struct KArgs {
     Tables t;
+    TablesX tx;
     double lin[4];             // rotation/skew matrix (lin_mat)
     double dx_m; int32_t dx_e; int32_t pad1;
     double drift_m[2]; int32_t drift_e[2];
@@ -43,12 +53,18 @@ struct KArgs {
     unsigned long long *work;
 };
 
-struct LaunchCfg { int kind, degree, flavor, xr; };
+struct LaunchCfg { int kind, degree, flavor, xr, precision; };
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
 hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
                             double kc, double eps, double *M, double *r, int32_t stages,
                             hipStream_t stream);
 int tile_kernel_blocks(const LaunchCfg &cfg);
+
+// FSG_PREC_FP32X kernels (fsg_fp32x.hip)
+constexpr size_t XS_BYTES = 12;
+hipError_t build_xs_tables(const Tables &t, int64_t bla_len, TablesX &x, hipStream_t stream);
+hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
+bool f32_supported(const LaunchCfg &cfg);
 
 }  // namespace fsg

@@ -15,6 +15,7 @@
 //                  burning_ship.py:12-60, 440-859
 //   BLA: numba_make_BLA* / combine_BLA* / ref_BLA_get, perturbation.py:1819-2170
 #include "models.hpp"
+#include "tile_loop.hpp"
 
 namespace fsg {
 
@@ -45,8 +46,6 @@ FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
     xc cx = renorm_c(a.dx_m * x1, a.dx_m * y1, a.dx_e);
     return cx + mkc(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
 }
-
-struct Counters { unsigned long long plain, bla, bla_it, reb, tries; };
 
 FSG_HD cplx zc() { return mk(0., 0.); }
 FSG_HD xc zx() { return mkc(0., 0., 0); }
@@ -422,53 +421,6 @@ struct BsXr {
         a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
     }
 };
-
-// --------------------------------------------------------------------------
-// The persistent tile kernel
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-template <class Model>
-__global__ void __launch_bounds__(256) k_tile(const KArgs a) {
-    typename Model::State s;
-    Counters k = {0, 0, 0, 0, 0};
-    int64_t idx = 0;
-    bool need = true;
-    for (;;) {
-        if (need) {
-            // lanes that need a pixel take consecutive indices from one atomic
-            const unsigned long long mask = __ballot(1);
-            const int leader = __ffsll((long long)mask) - 1;
-            const unsigned rank = __builtin_amdgcn_mbcnt_hi(
-                (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-            unsigned long long base = 0;
-            if ((int)__lane_id() == leader)
-                base = atomicAdd(a.work, (unsigned long long)__popcll(mask));
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)base, leader);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(base >> 32), leader);
-            idx = (int64_t)(((unsigned long long)hi << 32) | lo) + rank;
-            if (idx >= a.npts) break;
-            Model::init(s, a, idx);
-            need = false;
-        }
-        const int reason = Model::step(s, a, k);
-        if (reason >= 0) {
-            Model::finish(s, a, idx, reason);
-            need = true;
-        }
-    }
-    // all lanes have left the loop: one atomic per wave per counter
-    unsigned long long c0 = wave_sum(k.plain), c1 = wave_sum(k.bla);
-    unsigned long long c2 = wave_sum(k.bla_it), c3 = wave_sum(k.reb), c4 = wave_sum(k.tries);
-    if (__lane_id() == 0) {
-        atomicAdd(a.work + 1, c0); atomicAdd(a.work + 2, c1);
-        atomicAdd(a.work + 3, c2); atomicAdd(a.work + 4, c3);
-        atomicAdd(a.work + 5, c4);
-    }
-}
 
 // --------------------------------------------------------------------------
 // BLA build: level 0 + three compressed levels per block of 8 iterations,

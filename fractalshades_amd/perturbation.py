@@ -14,7 +14,7 @@ import numpy as np
 
 from . import settings
 from . import mpmath_loop as fsFP
-from ._native import GpuContext, FSG_KIND_M2, FSG_KIND_MN, FSG_KIND_BS, FSG_PREC_FP64
+from ._native import GpuContext, FSG_PREC_FP64, FSG_PREC_FP32X
 from .core import Fractal
 from .utils import zoom_options
 from .xrange import mpf_to_xr, mpc_to_xr, xr_to_float
@@ -22,6 +22,7 @@ from .xrange import mpf_to_xr, mpc_to_xr, xr_to_float
 logger = logging.getLogger(__name__)
 
 REF_ORDER_NONE = 1 << 62   # perturbation.py:455-456
+PRECISIONS = {"fp64": FSG_PREC_FP64, "fp32x": FSG_PREC_FP32X}
 
 
 class CycleIndepArgs:
@@ -47,6 +48,10 @@ class CycleIndepArgs:
             self._ctx.close()
             self._ctx = None
 
+    def with_fields(self, **over):
+        """A copy with some descriptor fields replaced (its own context)."""
+        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted)
+
 
 class PerturbationFractal(Fractal):
     """Escape-time fractal z_{n+1} = f(z_n) + c computed by perturbation
@@ -56,7 +61,6 @@ class PerturbationFractal(Fractal):
         super().__init__(directory)
         self._FP_params = None
         self._Zn_path = None
-        self.precision_mode = FSG_PREC_FP64
 
     # ---- zoom ----------------------------------------------------------------
     @zoom_options
@@ -290,7 +294,7 @@ class PerturbationFractal(Fractal):
             "epsilon_stationnary": float(spec.get("epsilon_stationnary", 0.) or 0.),
             "calc_dzndc": int(spec["calc_dzndc"]),
             "interior_detect": int(spec.get("calc_dzndz", False)),
-            "precision": int(spec.get("precision", self.precision_mode)),
+            "precision": int(spec.get("precision", PRECISIONS[settings.gpu_precision])),
         }
         arrays = {
             "Zn_path": pd["Zn_path"].view(np.float64),

@@ -1,0 +1,98 @@
+"""Post-processing formulas on the raw tile results: continuous iteration
+count (potential) and distance estimate (DEM).
+
+* continuous iteration: Continuous_iter_pp with kind "infinity"
+  (src/fractalshades/postproc.py:352-406, formula 1001-1009):
+  nu_frac = -log(log|z_n k| / log(M k)) / log d,  k = |a_d|^(1/(d-1)),
+  then the fractional part is folded into (-1, 0] and n adjusted;
+* DEM: DEM_pp (postproc.py:684-731): |z_n| ln|z_n| / |dz_n/dc|; for the
+  burning ship |dz_n/dc| is the larger singular value of the Jacobian,
+  (hypot(a + d, b - c) + hypot(a - d, b + c)) / 2.
+
+Computed in float64 (the reference's default post-processing dtype is
+float32, settings.py:82; the parity checks need float64).  Also: the error
+report of the binary32 form against the binary64 drop-in.
+"""
+import numpy as np
+
+
+def continuous_iter(zn, stop_iter, d=2, a_d=1., M=1000., floor_iter=0):
+    """Continuous iteration number nu for every pixel."""
+    zn = np.asarray(zn, dtype=np.complex128)
+    n = np.asarray(stop_iter).reshape(-1).astype(np.int64)
+    k = np.abs(a_d) ** (1. / (d - 1.))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nu_frac = -(np.log(np.log(np.abs(zn * k)) / np.log(M * k)) / np.log(d))
+    nu_div, nu_mod = np.divmod(-nu_frac, 1.)
+    nu_frac = -nu_mod
+    with np.errstate(invalid="ignore"):
+        n = n - nu_div.astype(np.int64)
+    return (n - floor_iter) + nu_frac
+
+
+def dem(zn, dzndc=None, jac=None):
+    """Distance estimate |z| ln|z| / |dz/dc| (holomorphic: dzndc; burning
+    ship: jac = (dXdA, dXdB, dYdA, dYdB))."""
+    abs_zn = np.abs(np.asarray(zn))
+    if jac is not None:
+        a, b, c, d = (np.asarray(v, dtype=np.float64) for v in jac)
+        Q = np.hypot(a + d, b - c)
+        R = np.hypot(a - d, b + c)
+        abs_d = 0.5 * (Q + R)
+    else:
+        abs_d = np.abs(np.asarray(dzndc))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return abs_zn * np.log(abs_zn) / abs_d
+
+
+def fields_of(Z, codes, holomorphic=True):
+    """Split a raw Z array into named fields, following the codes."""
+    names = codes[0]
+    if holomorphic:
+        return {k: Z[i] for i, k in enumerate(names)}
+    out = {k: Z[i] for i, k in enumerate(names)}
+    out["zn"] = Z[0] + 1j * Z[1]
+    return out
+
+
+def error_report(test, ref, holomorphic=True, d=2, M=1000.):
+    """Per-output error of a result against a reference result of the same
+    frame.  Each argument is (Z, U, stop_reason, stop_iter) in the raw
+    layout.  Returns counts, mismatch fractions and relative errors of the
+    continuous iteration (nu) and of the DEM on the escaped pixels."""
+    Zt, _, srt, sit = test
+    Zr, _, srr, sir = ref
+    sit, sir = sit.reshape(-1), sir.reshape(-1)
+    srt, srr = srt.reshape(-1), srr.reshape(-1)
+    n = sit.size
+    iter_mis = sit != sir
+    reason_mis = srt != srr
+    esc = (srr == 1) & (srt == 1) & ~iter_mis
+    if holomorphic:
+        zt, zr = Zt[0], Zr[0]
+    else:
+        zt, zr = Zt[0] + 1j * Zt[1], Zr[0] + 1j * Zr[1]
+    out = {
+        "pixels": int(n),
+        "escape_count_mismatch": int(np.count_nonzero(iter_mis)),
+        "escape_count_mismatch_frac": float(np.count_nonzero(iter_mis) / max(n, 1)),
+        "escape_count_mismatch_pm1_frac": float(np.count_nonzero(np.abs(sit.astype(np.int64) - sir) > 1) / max(n, 1)),
+        "stop_reason_mismatch": int(np.count_nonzero(reason_mis)),
+        "non_finite_zn": int(np.count_nonzero(~np.isfinite(zt))),
+        "unfinished": int(np.count_nonzero(srt < 0)),
+    }
+    if np.any(esc):
+        nut = continuous_iter(zt[esc], sit[esc], d=d, M=M)
+        nur = continuous_iter(zr[esc], sir[esc], d=d, M=M)
+        rel = np.abs(nut - nur) / np.abs(nur)
+        out["nu_rel_err_median"] = float(np.median(rel))
+        out["nu_rel_err_p99"] = float(np.quantile(rel, 0.99))
+        out["nu_rel_err_max"] = float(rel.max())
+        if holomorphic and Zr.shape[0] > 1:
+            dt = dem(zt[esc], Zt[-1][esc])
+            dr = dem(zr[esc], Zr[-1][esc])
+            rel = np.abs(dt - dr) / np.abs(dr)
+            out["dem_rel_err_median"] = float(np.median(rel))
+            out["dem_rel_err_p99"] = float(np.quantile(rel, 0.99))
+            out["dem_rel_err_max"] = float(rel.max())
+    return out

@@ -15,3 +15,8 @@ inspect_calc: bool = False
 chunk_size: int = 200
 BLA_compression: int = 3
 postproc_dtype: str = "float32"
+
+# Arithmetic of the GPU delta loop: "fp64" is the drop-in (the reference's
+# binary64 / extended-range semantics, bit for bit); "fp32x" runs binary32
+# mantissas with an int32 exponent (faster, approximate).
+gpu_precision: str = "fp64"

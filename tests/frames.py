@@ -1,9 +1,13 @@
 """Test frames: the reference test suite's zoom locations
 (tests/test_perturbation.py of Fractalshades) at reduced image sizes and
 iteration caps, plus helpers to run them through the oracle."""
+import json
+import os
+
 import numpy as np
 
 import fractalshades_amd as fsa
+from fractalshades_amd import configs
 
 # name: (model, zoom kwargs, calc kwargs) -- locations from the reference
 # tests (test_perturbation.py line numbers in the comments)
@@ -29,8 +33,40 @@ FRAMES = {
 }
 
 
+# burning-ship family: locations of the reference's batch examples
+# (tests/golden/example_frames.json, written by make_example_frames.py)
+_EX = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                  "golden", "example_frames.json")))
+for _name, _e in _EX.items():
+    FRAMES[_name] = ("BS", dict(x=_e["x"], y=_e["y"], dx=_e["dx"], precision=_e["precision"],
+                               has_skew=True, skew_00=_e["skew_00"], skew_01=_e["skew_01"],
+                               skew_10=_e["skew_10"], skew_11=_e["skew_11"]),
+                     dict(max_iter=min(_e["max_iter"], 20000)))
+    FRAMES[_name + "_flavor"] = _e["flavor"]
+
+# z^d + c: seeded boundary-descent centres (fractalshades_amd.configs)
+MN_TEST = {d: configs.Workload(f"T_MN{d}", "MN", "1e-14", 64, 64, 5000, 40, degree=d,
+                               period_range=(10, 3000), seed=d) for d in (3, 4, 6)}
+_mn_centers = {}
+
+
+def mn_frame(d, nx=48, xr=False, **calc_over):
+    if d not in _mn_centers:
+        _mn_centers[d] = configs.seeded_center(MN_TEST[d])
+    x, y, _, _ = _mn_centers[d]
+    f = fsa.Perturbation_mandelbrot_N("/tmp/fsa_test", d)
+    f.zoom(precision=40, x=x, y=y, dx="1e-14", nx=nx, xy_ratio=1.0, theta_deg=0.)
+    kw = dict(calc_name="t", subset=None, max_iter=5000, M_divergence=1.e3,
+              epsilon_stationnary=1.e-3, BLA_eps=1.e-6, interior_detect=True)
+    kw.update(calc_over)
+    f.calc_std_div(**kw)
+    return f
+
+
 def make(name, nx=48, model_kw=None, **calc_over):
     model, zoom_kw, calc_kw = FRAMES[name]
+    if model == "BS" and model_kw is None:
+        model_kw = {"flavor": FRAMES[name + "_flavor"]}
     if model == "M2":
         f = fsa.Perturbation_mandelbrot("/tmp/fsa_test")
     elif model == "MN":

@@ -73,3 +73,30 @@ def test_workload_subsample_bitexact(name):
     sub = tuple(np.ascontiguousarray(a[..., sel]) for a in got)
     assert_same(sub, ref)
     assert np.all(got[2] >= 0)
+
+
+@pytest.mark.parametrize("d", [3, 4, 6])
+@pytest.mark.parametrize("xr", [0, 1])
+def test_mn_bitexact(d, xr):
+    f = frames.mn_frame(d, nx=64)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case Mn d={d} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+
+
+@pytest.mark.parametrize("name,flavor,xr", [
+    ("BS_ex14", None, 0), ("BS_ex14", None, 1), ("PBS_ex18", None, 0),
+    ("SF_ex22", None, 1), ("BS_ex14", "Celtic", 0), ("BS_ex14", "Buffalo", 1),
+])
+def test_bs_bitexact(name, flavor, xr):
+    f = frames.make(name, nx=48, model_kw={"flavor": flavor} if flavor else None,
+                    max_iter=6000)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case {name} {flavor} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
