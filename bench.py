@@ -26,6 +26,25 @@ import numpy as np
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector / matrix peak
 
+# HBM bytes per launch of the binary64 tile kernel on workload C2, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
+# on a plain 256 MiB copy (`bench.py --calib-copy`): FETCH x2.0, WRITE x1.0.
+# Summary and commands: profiles/k_tile_C2_2048sq_1e-300.md.
+MEASURED_TRAFFIC = {("C2", 0): 373.2e6}
+
+
+def calib_copy():
+    """Plain device copy of known size (256 MiB, three times) against which
+    the FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": a.numel() * 8, "copies": 3}))
+
 
 def flops_per_unit(fields):
     """Algorithmic flops (add/mul = 1, sqrt = 1) per plain step, per BLA jump,
@@ -72,14 +91,96 @@ def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
     }
 
 
+class Leg:
+    """One workload x precision on the GPU: device buffers + timed steps."""
+
+    def __init__(self, f, cia, precision, torch, dev):
+        from fractalshades_amd._native import work_bytes
+        self.f = f
+        self.cia = cia if precision == cia.fields["precision"] else cia.with_fields(precision=precision)
+        self.ctx = self.cia.ctx
+        self.torch = torch
+        fields = self.cia.fields
+        self.fields = fields
+        c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
+        self.c_pix = c_pix
+        n = self.n = c_pix.size
+        if fields["kind"] == 2:
+            self.nz = 2 + (4 if fields["calc_dzndc"] else 0)
+            z_words = self.nz * n
+        else:
+            self.nz = 1 + int(bool(fields["calc_dzndc"])) + int(bool(fields["interior_detect"]))
+            z_words = 2 * self.nz * n
+        self.d_pix = torch.from_numpy(c_pix.view(np.float64)).to(dev)
+        self.d_Z = torch.empty(z_words, dtype=torch.float64, device=dev)
+        self.d_U = torch.empty(n, dtype=torch.int32, device=dev)
+        self.d_sr = torch.empty(n, dtype=torch.int8, device=dev)
+        self.d_si = torch.empty(n, dtype=torch.int32, device=dev)
+        self.d_work = torch.zeros(work_bytes() // 8, dtype=torch.int64, device=dev)
+        self.stream = torch.cuda.current_stream(dev)
+
+    def step(self):
+        self.ctx.tile_device(self.d_pix.data_ptr(), self.n, self.d_Z.data_ptr(), self.d_U.data_ptr(),
+                             self.d_sr.data_ptr(), self.d_si.data_ptr(), self.d_work.data_ptr(),
+                             self.stream.cuda_stream)
+
+    def run(self, steps, warmup):
+        from fractalshades_amd._native import work_stats
+        torch = self.torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(self.stream)
+        for _ in range(steps):
+            self.step()
+        ev1.record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        self.ms_per_step = wall * 1e3 / max(steps, 1)
+        self.kernel_ms = ev0.elapsed_time(ev1) / max(steps, 1)
+        self.pix_iters = int(self.d_si.to(torch.int64).sum().item())
+        self.reasons = torch.bincount(self.d_sr.to(torch.int64) + 1, minlength=4).cpu().tolist()
+        self.stats = work_stats(self.d_work.data_ptr())
+        self.value = self.pix_iters / (self.ms_per_step * 1e-3)
+        fp, fb, fl, fr = flops_per_unit(self.fields)
+        st = self.stats
+        self.flops = st["plain_steps"] * fp + st["bla_steps"] * fb + st["bla_lookups"] * fl + st["rebases"] * fr
+        self.tflops = self.flops / (self.kernel_ms * 1e-3) / 1e12
+        self.fpu = (fp, fb, fl, fr)
+        return self
+
+    def host_results(self):
+        n = self.n
+        Z = self.d_Z.cpu().numpy()
+        Z = Z.reshape(self.nz, n) if self.fields["kind"] == 2 else Z.view(np.complex128).reshape(self.nz, n)
+        return (Z, self.d_U.cpu().numpy().reshape(1, n), self.d_sr.cpu().numpy().reshape(1, n),
+                self.d_si.cpu().numpy().reshape(1, n))
+
+    def summary(self):
+        return {"value": self.value, "ms_per_step": self.ms_per_step, "kernel_ms": self.kernel_ms,
+                "pixel_iters_per_step": self.pix_iters, "fp64_equiv_tflops": self.tflops,
+                "counts": self.stats,
+                "stop_reason_hist": dict(zip(["-1", "max_iter", "divergence", "stationnary"], self.reasons))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="C2")
+    ap.add_argument("--deep", default="C2_2000", help="second frame (1e-2000); '' to skip")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 256 MiB device copy (PMC counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        calib_copy()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; "
@@ -89,100 +190,69 @@ def main():
 
     import torch
     from fractalshades_amd import configs
-    from fractalshades_amd._native import work_bytes, work_stats
+    from fractalshades_amd.postproc import error_report
 
     dev = torch.device("cuda:0")
+    frames = {}
+    main_leg = None
+    for wname in [args.workload] + ([args.deep] if args.deep else []):
+        w = configs.WORKLOADS[wname]
+        t0 = time.perf_counter()
+        f = configs.build(w)
+        cia = f._calc_data["bench"]["cycle_indep_args"]
+        prep_s = time.perf_counter() - t0
+        leg64 = Leg(f, cia, 0, torch, dev).run(args.steps, args.warmup)
+        entry = {"width": w.width, "pixels": leg64.n, "host_prep_s": round(prep_s, 2),
+                 "ref_len": cia.fields["ref_len"], "ref_order": f.FP_params["order"],
+                 "ref_div_iter": f.FP_params["div_iter"], "xr_mode": cia.fields["xr_mode"],
+                 "fp64": leg64.summary()}
+        if not args.no_fp32x:
+            leg32 = Leg(f, cia, 1, torch, dev).run(args.steps, args.warmup)
+            entry["fp32x"] = leg32.summary()
+            entry["fp32x"]["speedup_vs_fp64"] = leg32.value / leg64.value
+            entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), leg64.host_results())
+            del leg32
+        frames[wname] = entry
+        if main_leg is None:
+            main_leg = leg64
+        else:
+            del leg64
+    leg = main_leg
+    fp, fb, fl, fr = leg.fpu
     w = configs.WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    f = configs.build(w)
-    cia = f._calc_data["bench"]["cycle_indep_args"]
-    ctx = cia.ctx
-    prep_s = time.perf_counter() - t0
-    fields = cia.fields
-    c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
-    n = c_pix.size
-    kind = fields["kind"]
-    if kind == 2:
-        nz = 2 + (4 if fields["calc_dzndc"] else 0)
-        z_words = nz * n
-    else:
-        nz = 1 + int(bool(fields["calc_dzndc"])) + int(bool(fields["interior_detect"]))
-        z_words = 2 * nz * n
-
-    d_pix = torch.from_numpy(c_pix.view(np.float64)).to(dev)
-    d_Z = torch.empty(z_words, dtype=torch.float64, device=dev)
-    d_U = torch.empty(n, dtype=torch.int32, device=dev)
-    d_sr = torch.empty(n, dtype=torch.int8, device=dev)
-    d_si = torch.empty(n, dtype=torch.int32, device=dev)
-    d_work = torch.zeros(work_bytes() // 8, dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream(dev)
-
-    def step():
-        ctx.tile_device(d_pix.data_ptr(), n, d_Z.data_ptr(), d_U.data_ptr(), d_sr.data_ptr(),
-                        d_si.data_ptr(), d_work.data_ptr(), stream.cuda_stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    dev_ms = ev0.elapsed_time(ev1) / max(args.steps, 1)
-
-    pix_iters = int(d_si.to(torch.int64).sum().item())
-    reasons = torch.bincount(d_sr.to(torch.int64) + 1, minlength=4).cpu().tolist()
-    st = work_stats(d_work.data_ptr())
-    fp, fb, fl, fr = flops_per_unit(fields)
-    flops = st["plain_steps"] * fp + st["bla_steps"] * fb + st["bla_lookups"] * fl + st["rebases"] * fr
-    achieved = flops / (dev_ms * 1e-3) / 1e12
-    ms_per_step = wall * 1e3 / max(args.steps, 1)
-    value = pix_iters / (ms_per_step * 1e-3)
-
     out = {
         "metric": "pixel-iters/s at 1e-300 & 1e-2000 (fp64 vs fp32+xexp); p50/p99 tile latency",
-        "value": value,
+        "value": leg.value,
         "unit": "pixel-iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": leg.ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+xexp" if fields["xr_mode"] else "f64",
+        "dtype": "f64+xexp" if leg.fields["xr_mode"] else "f64",
         "data": "synthetic (seeded nucleus-search frame)",
         "config": {
-            "workload": f"{w.name}: Perturbation_mandelbrot {f.nx}x{f.ny} px, width {w.width}, "
+            "workload": f"{w.name}: Perturbation_mandelbrot {leg.f.nx}x{leg.f.ny} px, width {w.width}, "
                         f"max_iter {w.max_iter}, interior_detect {w.interior_detect}, "
                         f"BLA_eps 1e-6, FP64 drop-in (Xrange deltas)",
-            "pixels": n, "pixel_iters_per_step": pix_iters,
-            "ref_len": fields["ref_len"], "ref_order": f.FP_params["order"],
-            "ref_div_iter": f.FP_params["div_iter"],
-            "stop_reason_hist": {"-1": reasons[0], "max_iter": reasons[1],
-                                 "divergence": reasons[2], "stationnary": reasons[3]},
-            "host_prep_s": round(prep_s, 2),
+            "pixels": leg.n, "pixel_iters_per_step": leg.pix_iters,
+            "frames": frames,
         },
         "roofline": {
             "bound": "valu_fp64",
-            "achieved": achieved,
+            "achieved": leg.tflops,
             "peak": FP64_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": dev_ms,
-            "counts": st,
+            "frac": leg.tflops / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": MEASURED_TRAFFIC.get((args.workload, 0)),
+            "kernel_ms": leg.kernel_ms,
             "flops_per_unit": {"plain": fp, "bla": fb, "lookup": fl, "rebase": fr},
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(fields, cia.arrays, c_pix)
+        out["cpu_baseline"] = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
     print(json.dumps(out), flush=True)
 
 

@@ -57,8 +57,12 @@ WORKLOADS = {
     # widened
     "C2": Workload("C2", "M2", "1e-300", 2048, 2048, 200000, 320, interior_detect=True,
                    period_range=(50, 2000)),
+    # at 1e-2000 the nucleus of the seeded minibrot (~width^2 in size) needs
+    # ~4000 digits: the reference's Newton at the frame's 2040 digits cannot
+    # converge and falls back to the image centre, which no_newton does at once
     "C2_2000": Workload("C2_2000", "M2", "1e-2000", 2048, 2048, 200000, 2040,
-                        interior_detect=True, period_range=(50, 20000)),
+                        interior_detect=True, period_range=(50, 20000),
+                        extra={"no_newton": True}),
     "C3_d3": Workload("C3_d3", "MN", "1e-800", 1920, 1080, 1000000, 850, degree=3,
                       period_range=(50, 5000)),
     "C3_d4": Workload("C3_d4", "MN", "1e-800", 1920, 1080, 1000000, 850, degree=4,
@@ -92,28 +96,34 @@ def descend_to_boundary(c0, target_log2, prec, nmax=20000, degree=2, max_steps=1
     """Walk from an exterior point c0 towards the set along the potential
     gradient: each step moves by half the distance estimate
     d = |z| ln|z| / |dz/dc| (escape radius 1e5), until log2(d) < target_log2.
-    Fixed point with `prec` fractional bits.  Returns (c, d) or None when
-    c0 does not escape."""
-    one = 1 << prec
-    R2 = (10 ** 10) << (2 * prec)
+    c is kept in fixed point with `prec` fractional bits; each step iterates
+    with only -log2(d) + 64 of them.  Returns (c, log2 d) or None when c0 does
+    not escape."""
+    one_f = 1 << prec
     cx = int(mpmath.ldexp(mpmath.mpf(c0.real), prec))
     cy = int(mpmath.ldexp(mpmath.mpf(c0.imag), prec))
+    d_log2 = 0.
     for _ in range(max_steps):
+        p = min(prec, max(64, int(-d_log2) + 64))
+        sh = prec - p
+        one = 1 << p
+        R2 = (10 ** 10) << (2 * p)
+        ccx, ccy = cx >> sh, cy >> sh
         x = y = 0
         dxr, dxi = 0, 0
         escaped = False
         for _n in range(nmax):
             # dz <- d * z^(d-1) * dz + 1 ; z <- z^d + c
             if degree == 2:
-                dxr, dxi = ((2 * (x * dxr - y * dxi)) >> prec) + one, (2 * (x * dxi + y * dxr)) >> prec
-                x, y = ((x * x - y * y) >> prec) + cx, ((2 * x * y) >> prec) + cy
+                dxr, dxi = ((2 * (x * dxr - y * dxi)) >> p) + one, (2 * (x * dxi + y * dxr)) >> p
+                x, y = ((x * x - y * y) >> p) + ccx, ((2 * x * y) >> p) + ccy
             else:
                 pr, pi = one, 0
                 for _k in range(degree - 1):
-                    pr, pi = (pr * x - pi * y) >> prec, (pr * y + pi * x) >> prec
-                dxr, dxi = (degree * (pr * dxr - pi * dxi) >> prec) + one, degree * (pr * dxi + pi * dxr) >> prec
-                zr, zi = (pr * x - pi * y) >> prec, (pr * y + pi * x) >> prec
-                x, y = zr + cx, zi + cy
+                    pr, pi = (pr * x - pi * y) >> p, (pr * y + pi * x) >> p
+                dxr, dxi = (degree * (pr * dxr - pi * dxi) >> p) + one, degree * (pr * dxi + pi * dxr) >> p
+                zr, zi = (pr * x - pi * y) >> p, (pr * y + pi * x) >> p
+                x, y = zr + ccx, zi + ccy
             if x * x + y * y > R2:
                 escaped = True
                 break
@@ -123,24 +133,23 @@ def descend_to_boundary(c0, target_log2, prec, nmax=20000, degree=2, max_steps=1
         mr, er = _fmant(dxr)
         mi, ei = _fmant(dxi)
         e = max(er, ei)
-        dzm = complex(math.ldexp(mr, er - e), math.ldexp(mi, ei - e))   # dz = dzm * 2^(e - prec)
+        dzm = complex(math.ldexp(mr, er - e), math.ldexp(mi, ei - e))   # dz = dzm * 2^(e - p)
         az = abs(zf)
-        # distance estimate, as (mantissa, log2)
         dm = az * math.log(az) / abs(dzm)
-        d_log2 = math.log2(dm) - (e - prec)
+        d_log2 = math.log2(dm) - (e - p)
         if d_log2 < target_log2:
             return mpmath.mpc(mpmath.ldexp(cx, -prec), mpmath.ldexp(cy, -prec)), d_log2
-        # step -(ln|z| / 2) * z / dz
-        q = -(0.5 * math.log(az)) * zf / dzm          # times 2^(prec - e) in c units
-        sh = prec - e + prec                          # into fixed point
-        qr_m, qr_e = math.frexp(q.real)
-        qi_m, qi_e = math.frexp(q.imag)
-        def to_fixed(m, ex):
+        # step -(ln|z| / 2) * z / dz, i.e. q * 2^(p - e) in units of c
+        q = -(0.5 * math.log(az)) * zf / dzm
+        shift = prec + p - e                          # into prec-bit fixed point
+
+        def to_fixed(v):
+            m, ex = math.frexp(v)
             k = int(m * (1 << 53))
-            t = ex - 53 + sh
+            t = ex - 53 + shift
             return k << t if t >= 0 else k >> (-t)
-        cx += to_fixed(qr_m, qr_e)
-        cy += to_fixed(qi_m, qi_e)
+        cx += to_fixed(q.real)
+        cy += to_fixed(q.imag)
     return None
 
 
@@ -240,6 +249,7 @@ def build(w, nx=None, ny=None, max_iter=None, center=None, bla_eps=1.e-6):
     f = new_model(w)
     f.zoom(precision=w.digits, x=x, y=y, dx=w.width, nx=nx, xy_ratio=nx / ny,
            theta_deg=0.)
+    f.no_newton = bool(w.extra.get("no_newton", False))
     kw = dict(calc_name="bench", subset=None, max_iter=max_iter or w.max_iter,
               M_divergence=1.e3, BLA_eps=bla_eps)
     if w.model == "BS":

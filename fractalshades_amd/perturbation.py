@@ -135,7 +135,7 @@ class PerturbationFractal(Fractal):
     def _get_FP_orbit(self, c0, newton, order, max_newton):
         if c0 is None:
             c0 = self.x + 1j * self.y
-        if settings.no_newton or newton is None or newton == "None":
+        if settings.no_newton or getattr(self, "no_newton", False) or newton is None or newton == "None":
             self.compute_FP_orbit(c0, None)
             return
         if order is None:
