@@ -18,10 +18,13 @@ FSG_INTERRUPTED = 1
 
 # entry points declared in include/fsgpu.h
 EXPORTS = (
-    "fsg_abi_version", "fsg_ctx_create", "fsg_ctx_destroy", "fsg_tile",
+    "fsg_abi_version", "fsg_ctx_create", "fsg_ctx_create_shared", "fsg_ctx_destroy", "fsg_tile",
     "fsg_tile_device", "fsg_work_bytes", "fsg_work_stats",
     "fsg_ctx_read_table", "fsg_ctx_bla_shape", "fsg_last_error",
+    "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
+    "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
 )
+FSG_PENDING = 2
 
 
 class FsgError(RuntimeError):
@@ -63,6 +66,23 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerOpts(ctypes.Structure):
+    """fsg_server_opts (include/fsgpu.h)."""
+    _fields_ = [("lanes", ctypes.c_int32), ("chunk_px", ctypes.c_int32), ("hbm_budget", ctypes.c_int64)]
+
+
+class RequestInfo(ctypes.Structure):
+    """fsg_request_info (include/fsgpu.h)."""
+    _fields_ = [
+        ("status", ctypes.c_int32), ("chunks", ctypes.c_int32), ("footprint", ctypes.c_int64),
+        ("t_submit_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double),
+        ("t_first_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -86,6 +106,7 @@ def load(path=None):
         sig = {
             "fsg_abi_version": ([], ctypes.c_int),
             "fsg_ctx_create": ([ctypes.POINTER(RefDesc), ctypes.POINTER(vp)], ctypes.c_int),
+            "fsg_ctx_create_shared": ([ctypes.POINTER(RefDesc), vp, ctypes.POINTER(vp)], ctypes.c_int),
             "fsg_ctx_destroy": ([vp], ctypes.c_int),
             "fsg_tile": ([vp, vp, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "fsg_tile_device": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
@@ -94,6 +115,15 @@ def load(path=None):
             "fsg_ctx_read_table": ([vp, ctypes.c_int, vp, ctypes.c_size_t], ctypes.c_int),
             "fsg_ctx_bla_shape": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i32)], ctypes.c_int),
             "fsg_last_error": ([], ctypes.c_char_p),
+            "fsg_server_create": ([ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)], ctypes.c_int),
+            "fsg_server_destroy": ([vp], ctypes.c_int),
+            "fsg_server_footprint": ([vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
+            "fsg_server_clock": ([vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "fsg_server_submit": ([vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
+                                  ctypes.c_int),
+            "fsg_server_wait": ([vp, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)], ctypes.c_int),
+            "fsg_host_alloc": ([ctypes.c_size_t], vp),
+            "fsg_host_free": ([vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -121,7 +151,9 @@ def _ptr(a):
 class GpuContext:
     """A device-resident reference (fsg_ctx): orbit, derivative paths, BLA."""
 
-    def __init__(self, desc_fields, arrays):
+    def __init__(self, desc_fields, arrays, share=None):
+        """share: a GpuContext on the same orbit whose device orbit and BLA
+        coefficients this one reuses (fsg_ctx_create_shared)."""
         lib = load()
         # keep the input arrays alive until fsg_ctx_create has copied them
         self._arrays = arrays
@@ -140,7 +172,11 @@ class GpuContext:
         desc.n_xr = int(arrays["xr_index"].size)
         self.desc = desc
         handle = ctypes.c_void_p()
-        check(lib.fsg_ctx_create(ctypes.byref(desc), ctypes.byref(handle)))
+        if share is None:
+            check(lib.fsg_ctx_create(ctypes.byref(desc), ctypes.byref(handle)))
+        else:
+            check(lib.fsg_ctx_create_shared(ctypes.byref(desc), share.handle, ctypes.byref(handle)))
+        self._base = share            # the base outlives this context
         self._h = handle
         self.kind = desc.kind
 

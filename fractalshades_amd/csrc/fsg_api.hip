@@ -6,6 +6,7 @@
 #include "fsgpu.h"
 #include "models.hpp"
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -193,7 +194,7 @@ struct fsg_ctx {
     int32_t *dc_e = nullptr, *dz_e = nullptr;
     double *bs[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *bs_e[4] = {nullptr, nullptr, nullptr, nullptr};
-    int32_t *xr_idx = nullptr, *xr_e = nullptr;
+    int32_t *xr_idx = nullptr, *xr_e = nullptr, *xr_pos = nullptr;
     double *xr_m = nullptr;
     double *M = nullptr, *r = nullptr;
     int64_t bla_len = 0;
@@ -207,6 +208,14 @@ struct fsg_ctx {
     size_t buf_bytes = 0;
     unsigned long long *work = nullptr;
     std::vector<void *> owned;
+    // orbit, Xrange register and BLA coefficients: owned jointly by every
+    // context created on the same reference (fsg_ctx_create_shared)
+    struct Shared {
+        std::vector<void *> p;
+        ~Shared() { for (void *q : p) if (q) (void)hipFree(q); }
+    };
+    std::shared_ptr<Shared> shared;
+    uint64_t ref_hash = 0;
 
     ~fsg_ctx() {
         for (void *p : owned) if (p) (void)hipFree(p);
@@ -219,11 +228,38 @@ struct fsg_ctx {
 
 namespace {
 
-template <class T> int upload(fsg_ctx *c, T *&dst, const T *src, size_t n) {
+// FNV-1a of the reference data a shared context must agree on
+// (64-bit words, then the tail bytes)
+uint64_t fnv(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h ^= w; h *= 1099511628211ull;
+    }
+    for (; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+uint64_t ref_hash_of(const fsg_ref_desc *d) {
+    uint64_t h = 1469598103934665603ull;
+    const int32_t key[4] = {d->kind, d->kind == FSG_KIND_M2 ? 2 : d->degree, d->flavor, d->n_xr};
+    h = fnv(h, key, sizeof(key));
+    h = fnv(h, d->Zn_path, (size_t)d->ref_len * 16);
+    if (d->n_xr > 0) {
+        const bool holo = d->kind != FSG_KIND_BS;
+        h = fnv(h, d->xr_index, (size_t)d->n_xr * 4);
+        h = fnv(h, d->xr_mant, (size_t)d->n_xr * 16);
+        h = fnv(h, d->xr_exp, (size_t)d->n_xr * (holo ? 4 : 8));
+    }
+    return h;
+}
+
+template <class T> int upload(fsg_ctx *c, T *&dst, const T *src, size_t n, bool shared = false) {
     hipError_t err;
     dst = dalloc<T>(n, err);
     if (err != hipSuccess) return fail(FSG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(err));
-    c->owned.push_back(dst);
+    (shared ? c->shared->p : c->owned).push_back(dst);
     if (n) FSG_HIP(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
@@ -264,6 +300,7 @@ int fill_args(fsg_ctx *c, KArgs &a) {
     t.dc = c->dc; t.dc_e = c->dc_e; t.dz = c->dz; t.dz_e = c->dz_e;
     for (int q = 0; q < 4; q++) { t.bs[q] = c->bs[q]; t.bs_e[q] = c->bs_e[q]; }
     t.n_xr = c->d.n_xr; t.xr_idx = c->xr_idx; t.xr_m = c->xr_m; t.xr_e = c->xr_e;
+    t.xr_pos = c->d.n_xr ? c->xr_pos : nullptr;
     t.M = c->M; t.r = c->r; t.stages_bla = c->stages; t.bla_on = c->d.bla_on;
     a.tx = c->tx;
     for (int q = 0; q < 4; q++) a.lin[q] = c->d.lin_mat[q];
@@ -296,10 +333,11 @@ size_t z_bytes(const fsg_ctx *c, int64_t n) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8_t *sr, int32_t *si,
-           unsigned long long *work, hipStream_t stream, bool timed) {
+           unsigned long long *work, hipStream_t stream, bool timed, int64_t ld = -1, int max_blocks = 0) {
     KArgs a;
     fill_args(c, a);
-    a.pix = pix; a.npts = n; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si; a.work = work;
+    a.pix = pix; a.npts = n; a.ld = ld < 0 ? n : ld; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si;
+    a.work = work; a.max_blocks = max_blocks;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, work, WORK_WORDS);
     FSG_HIP(hipGetLastError());
     if (n == 0) return 0;
@@ -312,6 +350,23 @@ int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8
 
 }  // namespace
 
+// ---- internal entry points of the serving loop (fsg_serve.hip) ----
+namespace fsg {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+int ctx_holomorphic(const fsg_ctx *c) { return c->d.kind != FSG_KIND_BS; }
+int ctx_nz(const fsg_ctx *c) {
+    return c->d.kind == FSG_KIND_BS ? 2 + (c->d.calc_dzndc ? 4 : 0)
+                                    : 1 + (c->d.calc_dzndc ? 1 : 0) + (c->d.interior_detect ? 1 : 0);
+}
+size_t ctx_z_bytes(const fsg_ctx *c, int64_t n) { return z_bytes(c, n); }
+// pixels [0, n) of pix into columns [0, n) of outputs with row stride ld
+int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, double *Z, int32_t *U,
+                     int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks) {
+    return launch(c, pix, n, Z, U, sr, si, work, stream, false, ld, max_blocks);
+}
+size_t work_words() { return WORK_WORDS; }
+}  // namespace fsg
+
 extern "C" {
 
 int fsg_abi_version(void) { return FSG_ABI_VERSION; }
@@ -320,11 +375,23 @@ const char *fsg_last_error(void) { return g_err.c_str(); }
 
 size_t fsg_work_bytes(void) { return WORK_WORDS * sizeof(unsigned long long); }
 
-int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
+}  // extern "C"
+
+namespace {
+
+int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
     if (!out) return fail(FSG_E_INVALID, "null output pointer");
     *out = nullptr;
     int rc = validate(desc);
     if (rc) return rc;
+    const uint64_t h_ref = ref_hash_of(desc);
+    if (base) {
+        if (base->ref_hash != h_ref || base->L != desc->ref_len)
+            return fail(FSG_E_INVALID, "shared context: the orbit or its Xrange register differs from the base's");
+        int dev = -1;
+        if (hipGetDevice(&dev) != hipSuccess || dev != base->device)
+            return fail(FSG_E_INVALID, "shared context: the base lives on another device");
+    }
     fsg_ctx *c = new (std::nothrow) fsg_ctx();
     if (!c) return fail(FSG_E_NOMEM, "out of host memory");
     c->d = *desc;
@@ -343,12 +410,27 @@ int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
         return cleanup(fail(FSG_E_HIP, "stream/event creation failed"));
 
     const int64_t L = c->L;
-    if ((rc = upload(c, c->Zn, reinterpret_cast<const cplx *>(desc->Zn_path), (size_t)L))) return cleanup(rc);
     const int32_t nx = desc->n_xr;
     const int holo = desc->kind != FSG_KIND_BS;
-    if ((rc = upload(c, c->xr_idx, desc->xr_index, (size_t)nx))) return cleanup(rc);
-    if ((rc = upload(c, c->xr_m, desc->xr_mant, (size_t)2 * nx))) return cleanup(rc);
-    if ((rc = upload(c, c->xr_e, desc->xr_exp, (size_t)(holo ? nx : 2 * nx)))) return cleanup(rc);
+    c->ref_hash = h_ref;
+    if (base) {
+        c->shared = base->shared;
+        c->Zn = base->Zn; c->xr_idx = base->xr_idx; c->xr_m = base->xr_m; c->xr_e = base->xr_e;
+        c->xr_pos = base->xr_pos;
+    } else {
+        c->shared = std::make_shared<fsg_ctx::Shared>();
+        if ((rc = upload(c, c->Zn, reinterpret_cast<const cplx *>(desc->Zn_path), (size_t)L, true))) return cleanup(rc);
+        if ((rc = upload(c, c->xr_idx, desc->xr_index, (size_t)nx, true))) return cleanup(rc);
+        if ((rc = upload(c, c->xr_m, desc->xr_mant, (size_t)2 * nx, true))) return cleanup(rc);
+        if ((rc = upload(c, c->xr_e, desc->xr_exp, (size_t)(holo ? nx : 2 * nx), true))) return cleanup(rc);
+        if (nx) {
+            // dense slot table: one load per orbit point instead of a search
+            std::vector<int32_t> pos((size_t)L + 1, -1);
+            for (int32_t j = 0; j < nx; j++)
+                if (desc->xr_index[j] >= 0 && desc->xr_index[j] <= L) pos[(size_t)desc->xr_index[j]] = j;
+            if ((rc = upload(c, c->xr_pos, pos.data(), pos.size(), true))) return cleanup(rc);
+        }
+    }
 
     HostRef h{&c->d, c->L, c->div_iter, c->order};
     // the derivative recurrences read the host orbit and register
@@ -382,18 +464,26 @@ int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
         c->bla_len = 2 * comp;
         int dim = holo ? 4 : 8;
         hipError_t err;
-        c->M = dalloc<double>((size_t)(c->bla_len * dim), err);
-        if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc BLA"));
-        c->owned.push_back(c->M);
+        // the coefficients depend on the orbit only: shared with the base
+        // when it has them; the radii depend on kc and eps: per context
+        const bool reuse_M = base && base->M && base->bla_len == c->bla_len;
+        if (reuse_M) {
+            c->M = base->M;
+        } else {
+            c->M = dalloc<double>((size_t)(c->bla_len * dim), err);
+            if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc BLA"));
+            (base ? c->owned : c->shared->p).push_back(c->M);
+            if (hipMemsetAsync(c->M, 0, (size_t)(c->bla_len * dim) * sizeof(double), c->stream) != hipSuccess)
+                return cleanup(fail(FSG_E_HIP, "hipMemsetAsync BLA"));
+        }
         c->r = dalloc<double>((size_t)c->bla_len, err);
         if (err != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc BLA"));
         c->owned.push_back(c->r);
-        if (hipMemsetAsync(c->M, 0, (size_t)(c->bla_len * dim) * sizeof(double), c->stream) != hipSuccess ||
-            hipMemsetAsync(c->r, 0, (size_t)c->bla_len * sizeof(double), c->stream) != hipSuccess)
+        if (hipMemsetAsync(c->r, 0, (size_t)c->bla_len * sizeof(double), c->stream) != hipSuccess)
             return cleanup(fail(FSG_E_HIP, "hipMemsetAsync BLA"));
         double kc = to_std(mkf(desc->kc_mant, desc->kc_exp));
         hipError_t be = launch_bla_build(desc->kind, c->d.degree, desc->flavor, c->Zn, c->L, kc,
-                                         desc->bla_eps, c->M, c->r, c->stages, c->stream);
+                                         desc->bla_eps, c->M, c->r, c->stages, c->stream, reuse_M);
         if (be != hipSuccess) return cleanup(fail(FSG_E_HIP, std::string("BLA build: ") + hipGetErrorString(be)));
         if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(FSG_E_HIP, "BLA build sync"));
     }
@@ -424,6 +514,17 @@ int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) {
     c->owned.push_back(c->work);
     *out = c;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out) { return ctx_create(desc, nullptr, out); }
+
+int fsg_ctx_create_shared(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
+    if (!base) return fail(FSG_E_INVALID, "null base context");
+    return ctx_create(desc, base, out);
 }
 
 int fsg_ctx_destroy(fsg_ctx *ctx) {

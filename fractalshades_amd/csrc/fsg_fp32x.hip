@@ -116,17 +116,7 @@ hipError_t build_xs_tables(const Tables &t, int64_t bla_len, TablesX &x, hipStre
 
 // BLA stage selection of ref_BLA_get (perturbation.py:2116-2170), on |z|
 __device__ __forceinline__ int32_t bla_lookup_az(const Tables &t, double az, int32_t w, int32_t &idx) {
-    int32_t it = w >> 3;
-    int32_t tz = (it == 0) ? 31 : __builtin_ctz((uint32_t)it);
-    int32_t stg = min(3 + tz, t.stages_bla - 1);
-    int32_t invalid_step = t.first_invalid - w;
-    for (; stg >= 3; stg--) {
-        int32_t step = 1 << stg;
-        if (step >= invalid_step) continue;
-        int32_t k = 2 * it + (1 << (stg - 3)) - 1;
-        if (az < t.r[k]) { idx = k; return step; }
-    }
-    return 0;
+    return bla_find(t, az, w, idx);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,12 +140,13 @@ struct HoloF32 {
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
         const xs *Zx = (const xs *)a.tx.Zx, *DC = (const xs *)a.tx.DCx, *DZ = (const xs *)a.tx.DZx;
-        if (t.bla_on && (s.w & 7) == 0) {
+        while (t.bla_on && (s.w & 7) == 0) {
             const double az = ldexp((double)hypotf(s.z.x, s.z.y), max(s.z.e, -2000));
             int32_t idx;
             const int32_t st = bla_lookup_az(t, az, s.w, idx);
             k.tries++;
-            if (st != 0) {
+            if (st == 0) break;
+            {
                 const xs A = ((const xs *)a.tx.Ax)[idx], B = ((const xs *)a.tx.Bx)[idx];
                 s.n += st;
                 s.w += st;
@@ -163,7 +154,6 @@ struct HoloF32 {
                 if (a.calc_dc) s.d = A * s.d;
                 if (a.calc_dz) s.h = A * s.h;
                 k.bla++; k.bla_it += st;
-                return -1;
             }
         }
         s.n += 1; k.plain++;
@@ -212,7 +202,7 @@ struct HoloF32 {
     }
 
     __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.npts;
+        const int64_t n = a.ld;
         const cplx z = xs_to_d(s.z) + a.t.Zn[s.w];
         a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
         int f = 1;
@@ -248,16 +238,9 @@ static tile_fn select_f32(const LaunchCfg &c) {
 hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
     tile_fn f = select_f32(cfg);
     if (!f) return hipErrorInvalidValue;
-    int dev = 0, ncu = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return hipErrorInvalidValue;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return hipErrorInvalidValue;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), 256, 0) != hipSuccess)
-        return hipErrorInvalidValue;
-    int blocks = ncu * (per_cu < 1 ? 1 : per_cu);
-    int64_t need = (a.npts + 255) / 256;
-    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
-    hipLaunchKernelGGL(f, dim3(blocks), dim3(256), 0, stream, a);
+    const int fill = grid_fill_blocks(reinterpret_cast<const void *>(f));
+    if (fill <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(f, dim3(launch_blocks(fill, a)), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -16,9 +16,11 @@ struct Tables {
     const cplx *dz; const int32_t *dz_e;
     // Burning-ship Jacobian paths dX/da, dX/db, dY/da, dY/db [L]
     const double *bs[4]; const int32_t *bs_e[4];
-    // sparse Xrange register of orbit points below 1e-300 (sorted indices)
+    // sparse Xrange register of orbit points below 1e-300 (sorted indices);
+    // xr_pos[i] = register slot of orbit point i or -1 (null when empty)
     int32_t n_xr; int32_t pad0;
     const int32_t *xr_idx; const double *xr_m; const int32_t *xr_e;
+    const int32_t *xr_pos;
     // BLA tree: 4 doubles (A, B complex) or 8 doubles (2x2 A, 2x2 B) per entry
     const double *M; const double *r;
     int32_t stages_bla; int32_t bla_on;
@@ -45,9 +47,12 @@ struct KArgs {
     // pixels: complex128 screen offsets in units of dx ("c_pix")
     const double *pix;
     int64_t npts;
-    // outputs, reference layout: Z [n_Z, npts], U/stop_iter int32, stop_reason int8
+    // outputs, reference layout: Z [n_Z, ld] (row stride ld >= npts, so that
+    // a launch can fill a column range of a larger tile), U/stop_iter int32,
+    // stop_reason int8
     double *Z; int32_t *U; int8_t *stop_reason; int32_t *stop_iter;
-    int32_t nz; int32_t pad3;
+    int64_t ld;
+    int32_t nz; int32_t max_blocks;   // grid cap: 0 fill the GPU, > 0 blocks, -k 1/k of the GPU
     // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
     // rebase, bla lookups)
     unsigned long long *work;
@@ -56,10 +61,14 @@ struct KArgs {
 struct LaunchCfg { int kind, degree, flavor, xr, precision; };
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
+// radii_only: M already holds the coefficients (shared with another
+// context of the same orbit); write the radii r only
 hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
                             double kc, double eps, double *M, double *r, int32_t stages,
-                            hipStream_t stream);
+                            hipStream_t stream, bool radii_only = false);
 int tile_kernel_blocks(const LaunchCfg &cfg);
+int grid_fill_blocks(const void *kernel);          // blocks filling the GPU at full occupancy
+int launch_blocks(int fill, const KArgs &a);       // capped by max_blocks and by the pixel count
 
 // FSG_PREC_FP32X kernels (fsg_fp32x.hip)
 constexpr size_t XS_BYTES = 12;

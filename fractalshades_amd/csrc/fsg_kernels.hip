@@ -16,27 +16,17 @@
 //   BLA: numba_make_BLA* / combine_BLA* / ref_BLA_get, perturbation.py:1819-2170
 #include "models.hpp"
 #include "tile_loop.hpp"
+#include "xl.hpp"
+
+#include <mutex>
+#include <unordered_map>
 
 namespace fsg {
 
-// --------------------------------------------------------------------------
-// BLA lookup (ref_BLA_get, perturbation.py:2116-2170).  The tree keeps the
-// stage s >= 3 jump starting at iteration w (a multiple of 2^s) at index
-// 2 (w >> 3) + 2^(s-3) - 1.  Returns the jump length, 0 if no stage is valid.
+// BLA lookup on binary64 |z| (ref_BLA_get, perturbation.py:2116-2170)
 __device__ __forceinline__ int32_t bla_lookup(const Tables &t, double zre, double zim,
                                               int32_t w, int32_t &idx) {
-    int32_t it = w >> 3;
-    int32_t tz = (it == 0) ? 31 : __builtin_ctz((uint32_t)it);
-    int32_t stg = min(3 + tz, t.stages_bla - 1);
-    int32_t invalid_step = t.first_invalid - w;
-    double az = hypot2(zre, zim);
-    for (; stg >= 3; stg--) {
-        int32_t step = 1 << stg;
-        if (step >= invalid_step) continue;
-        int32_t k = 2 * it + (1 << (stg - 3)) - 1;
-        if (az < t.r[k]) { idx = k; return step; }
-    }
-    return 0;
+    return bla_find(t, hypot2(zre, zim), w, idx);
 }
 
 // ---- pixel -> c (ref_path_c_from_pix + lin_proj_impl, perturbation.py:2214-2230, 2665-2672)
@@ -50,14 +40,10 @@ FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
 FSG_HD cplx zc() { return mk(0., 0.); }
 FSG_HD xc zx() { return mkc(0., 0., 0); }
 
-// register lookup: index into the sorted Xrange register, -1 if absent
-__device__ __forceinline__ int32_t xr_find(const Tables &t, int32_t i) {
-    int32_t lo = 0, hi = t.n_xr;
-    while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
-        if (t.xr_idx[mid] < i) lo = mid + 1; else hi = mid;
-    }
-    return (lo < t.n_xr && t.xr_idx[lo] == i) ? lo : -1;
+__device__ __forceinline__ xl c_from_pix_xl(const KArgs &a, double px, double py) {
+    const double x1 = a.lin[0] * px + a.lin[1] * py;
+    const double y1 = a.lin[2] * px + a.lin[3] * py;
+    return xl_of(a.dx_m * x1, a.dx_m * y1, a.dx_e) + xl_of(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
 }
 
 // --------------------------------------------------------------------------
@@ -74,11 +60,12 @@ struct HoloStd {
 
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
-        if (t.bla_on && (s.w & 7) == 0) {
+        while (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.z.re, s.z.im, s.w, idx);
             k.tries++;
-            if (st != 0) {
+            if (st == 0) break;
+            {
                 const double *Mb = t.M + 4 * idx;
                 cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
                 s.n += st;
@@ -87,7 +74,6 @@ struct HoloStd {
                 if (a.calc_dc) s.d = A * s.d;
                 if (a.calc_dz) s.h = A * s.h;
                 k.bla++; k.bla_it += st;
-                return -1;
             }
         }
         s.n += 1; k.plain++;
@@ -124,7 +110,7 @@ struct HoloStd {
     }
 
     __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.npts;
+        const int64_t n = a.ld;
         cplx z = s.z + a.t.Zn[s.w];
         a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
         int f = 1;
@@ -137,29 +123,39 @@ struct HoloStd {
     }
 };
 
-// Holomorphic models, Xrange deltas (dx < 1e-300)
+// Holomorphic models, Xrange deltas (dx < 1e-300).  The deltas are kept in
+// the lazily normalised extended-range form of xl.hpp: every result is
+// renormalised without branches, which rounds exactly like the reference's
+// Xrange record (a rescaling by a power of two is exact).
 template <int KIND, int DEG>
 struct HoloXr {
-    struct State { xc z, d, h, c, ox; cplx zs; int32_t w, n; bool dyn, nul, oxr; };
+    struct State { xl z, d, h, c; cplx zs; int32_t w, n; bool dyn, nul; };
 
-    __device__ static xc DC(const Tables &t, int32_t i) { cplx m = t.dc[i]; return mkc(m.re, m.im, t.dc_e[i]); }
-    __device__ static xc DZ(const Tables &t, int32_t i) { cplx m = t.dz[i]; return mkc(m.re, m.im, t.dz_e[i]); }
-    __device__ static xc REG(const Tables &t, int32_t j) { return mkc(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]); }
+    __device__ static xl DC(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xl_of(m.re, m.im, t.dc_e[i]); }
+    __device__ static xl DZ(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xl_of(m.re, m.im, t.dz_e[i]); }
+    // orbit point i: the register's Xrange value where the binary64 orbit
+    // underflows, the binary64 value elsewhere
+    __device__ static xl ZR(const Tables &t, int32_t i) {
+        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
+        if (j >= 0) return xl_of(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]);
+        return xl_of(t.Zn[i]);
+    }
 
     __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        s.c = c_from_pix(a, a.pix[2 * i], a.pix[2 * i + 1]);
-        s.z = zx(); s.d = zx(); s.h = zx(); s.ox = zx();
+        s.c = c_from_pix_xl(a, a.pix[2 * i], a.pix[2 * i + 1]);
+        s.z = xl_zero(); s.d = xl_zero(); s.h = xl_zero();
         s.zs = zc();
-        s.w = 0; s.n = 0; s.dyn = true; s.nul = false; s.oxr = false;
+        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
     }
 
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
-        if (t.bla_on && (s.w & 7) == 0) {
+        while (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.zs.re, s.zs.im, s.w, idx);
             k.tries++;
-            if (st != 0) {
+            if (st == 0) break;
+            {
                 const double *Mb = t.M + 4 * idx;
                 cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
                 s.n += st;
@@ -169,16 +165,11 @@ struct HoloXr {
                 if (a.calc_dc) s.d = s.d * A;
                 if (a.calc_dz) s.h = s.h * A;
                 k.bla++; k.bla_it += st;
-                return -1;
             }
         }
         s.n += 1; k.plain++;
-        xc Zr;
-        {
-            int32_t j = t.n_xr ? xr_find(t, s.w) : -1;
-            Zr = (j >= 0) ? REG(t, j) : to_xc(t.Zn[s.w]);
-        }
-        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zx() : DC(t, s.w));
+        const xl Zr = ZR(t, s.w);
+        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xl_zero() : DC(t, s.w));
         if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ(t, 0) : DZ(t, s.w));
         s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
         s.zs = to_std(s.z);
@@ -186,20 +177,14 @@ struct HoloXr {
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;
         if (a.calc_dz) {
-            xc nx = s.nul ? DZ(t, 0) : (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
+            xl nx = s.nul ? DZ(t, 0) : (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
             if (lt(abs2(s.h + nx), a.eps2)) return 2;
         }
-        if (t.n_xr) {
-            int32_t j = xr_find(t, s.w);
-            s.oxr = (j >= 0);
-            if (j >= 0) s.ox = REG(t, j);
-        }
-        const cplx Zn1 = t.Zn[s.w];
-        const cplx ZZ = s.zs + Zn1;
+        const cplx ZZ = s.zs + t.Zn[s.w];
         if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
         if (s.w >= t.div_iter - 1) {
             s.zs = ZZ;
-            s.z = to_xc(ZZ);
+            s.z = xl_of(ZZ);
             if (a.calc_dc) s.d = s.d + DC(t, s.w);
             if (a.calc_dz) {
                 if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
@@ -211,7 +196,7 @@ struct HoloXr {
         }
         s.dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
         if (s.dyn) {
-            xc ZZx = s.oxr ? s.z + s.ox : s.z + Zn1;
+            const xl ZZx = s.z + ZR(t, s.w);
             if (le(abs2(ZZx), abs2(s.z))) {
                 s.z = ZZx;
                 s.zs = to_std(ZZx);
@@ -228,7 +213,7 @@ struct HoloXr {
     }
 
     __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.npts;
+        const int64_t n = a.ld;
         cplx z = to_std(s.z) + a.t.Zn[s.w];
         a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
         int f = 1;
@@ -260,11 +245,12 @@ struct BsStd {
 
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
-        if (t.bla_on && (s.w & 7) == 0) {
+        while (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.x, s.y, s.w, idx);
             k.tries++;
-            if (st != 0) {
+            if (st == 0) break;
+            {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
                 double nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
@@ -278,7 +264,6 @@ struct BsStd {
                     s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
                 }
                 k.bla++; k.bla_it += st;
-                return -1;
             }
         }
         s.n += 1; k.plain++;
@@ -311,7 +296,7 @@ struct BsStd {
     }
 
     __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.npts;
+        const int64_t n = a.ld;
         const cplx R = a.t.Zn[s.w];
         a.Z[i] = s.x + R.re; a.Z[n + i] = s.y + R.im;
         if (a.calc_dc)
@@ -320,58 +305,58 @@ struct BsStd {
     }
 };
 
-// Burning-ship family, Xrange deltas
+// Burning-ship family, Xrange deltas (xl.hpp reals)
 template <int FL>
 struct BsXr {
-    struct State { xf x, y, P[4], a, b, ox, oy; double xs, ys; int32_t w, n; bool dyn, oxr; };
+    struct State { xlf x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; };
 
-    __device__ static xf PATH(const Tables &t, int q, int32_t i) { return mkf(t.bs[q][i], t.bs_e[q][i]); }
+    __device__ static xlf PATH(const Tables &t, int q, int32_t i) { return xlf_of(t.bs[q][i], t.bs_e[q][i]); }
+    __device__ static void ZR(const Tables &t, int32_t i, xlf &X, xlf &Y) {
+        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
+        if (j >= 0) { X = xlf_of(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xlf_of(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
+        else { const cplx R = t.Zn[i]; X = xlf_of(R.re, 0); Y = xlf_of(R.im, 0); }
+    }
 
     __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        xc cx = renorm_c(a.dx_m * (a.lin[0] * a.pix[2 * i] + a.lin[1] * a.pix[2 * i + 1]),
-                         a.dx_m * (a.lin[2] * a.pix[2 * i] + a.lin[3] * a.pix[2 * i + 1]), a.dx_e);
-        s.a = mkf(cx.re, cx.e) + mkf(a.drift_m[0], a.drift_e[0]);
-        s.b = mkf(cx.im, cx.e) + mkf(a.drift_m[1], a.drift_e[1]);
-        const xf z0 = mkf(0., 0);
-        s.x = z0; s.y = z0; s.P[0] = s.P[1] = s.P[2] = s.P[3] = z0; s.ox = z0; s.oy = z0;
+        const double px = a.pix[2 * i], py = a.pix[2 * i + 1];
+        s.a = xlf_of(a.dx_m * (a.lin[0] * px + a.lin[1] * py), a.dx_e) + xlf_of(a.drift_m[0], a.drift_e[0]);
+        s.b = xlf_of(a.dx_m * (a.lin[2] * px + a.lin[3] * py), a.dx_e) + xlf_of(a.drift_m[1], a.drift_e[1]);
+        const xlf z0 = xlf_zero();
+        s.x = z0; s.y = z0; s.P[0] = s.P[1] = s.P[2] = s.P[3] = z0;
         s.xs = 0.; s.ys = 0.;
-        s.w = 0; s.n = 0; s.dyn = true; s.oxr = false;
+        s.w = 0; s.n = 0; s.dyn = true;
     }
 
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
-        if (t.bla_on && (s.w & 7) == 0) {
+        while (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
             int32_t st = bla_lookup(t, s.xs, s.ys, s.w, idx);
             k.tries++;
-            if (st != 0) {
+            if (st == 0) break;
+            {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
-                xf nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
-                xf ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
+                xlf nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
+                xlf ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
                 s.x = nx; s.y = ny;
                 s.xs = to_std(s.x); s.ys = to_std(s.y);
                 if (a.calc_dc) {
-                    xf p0 = M[0] * s.P[0] + M[1] * s.P[2];
-                    xf p1 = M[0] * s.P[1] + M[1] * s.P[3];
-                    xf p2 = M[2] * s.P[0] + M[3] * s.P[2];
-                    xf p3 = M[2] * s.P[1] + M[3] * s.P[3];
+                    xlf p0 = M[0] * s.P[0] + M[1] * s.P[2];
+                    xlf p1 = M[0] * s.P[1] + M[1] * s.P[3];
+                    xlf p2 = M[2] * s.P[0] + M[3] * s.P[2];
+                    xlf p3 = M[2] * s.P[1] + M[3] * s.P[3];
                     s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
                 }
                 k.bla++; k.bla_it += st;
-                return -1;
             }
         }
         s.n += 1; k.plain++;
-        xf X, Y;
-        {
-            int32_t j = t.n_xr ? xr_find(t, s.w) : -1;
-            if (j >= 0) { X = mkf(t.xr_m[2 * j], t.xr_e[2 * j]); Y = mkf(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
-            else { cplx R = t.Zn[s.w]; X = to_xf(R.re); Y = to_xf(R.im); }
-        }
+        xlf X, Y;
+        ZR(t, s.w, X, Y);
         if (a.calc_dc) {
-            const xf z0 = mkf(0., 0);
-            xf Rd[4] = {z0, z0, z0, z0};
+            const xlf z0 = xlf_zero();
+            xlf Rd[4] = {z0, z0, z0, z0};
             if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = PATH(t, q, s.w);
             bs_hess<FL>(s.x, s.y, s.P, X, Y, Rd);
         }
@@ -380,17 +365,12 @@ struct BsXr {
         if (s.n >= a.max_iter) return 0;
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
-        if (t.n_xr) {
-            int32_t j = xr_find(t, s.w);
-            s.oxr = (j >= 0);
-            if (j >= 0) { s.ox = mkf(t.xr_m[2 * j], t.xr_e[2 * j]); s.oy = mkf(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
-        }
         const cplx R1 = t.Zn[s.w];
         double XX = s.xs + R1.re, YY = s.ys + R1.im;
         if (XX * XX + YY * YY > a.M2) return 1;
         if (s.w >= t.div_iter - 1) {
             s.xs = XX; s.ys = YY;
-            s.x = to_xf(XX); s.y = to_xf(YY);
+            s.x = xlf_of(XX, 0); s.y = xlf_of(YY, 0);
             if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
             s.w = 0;
             k.reb++;
@@ -398,9 +378,9 @@ struct BsXr {
         }
         s.dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
         if (s.dyn) {
-            xf XXx, YYx;
-            if (s.oxr) { XXx = s.x + s.ox; YYx = s.y + s.oy; }
-            else { XXx = s.x + R1.re; YYx = s.y + R1.im; }
+            xlf X1, Y1;
+            ZR(t, s.w, X1, Y1);
+            const xlf XXx = s.x + X1, YYx = s.y + Y1;
             if (le(XXx * XXx + YYx * YYx, s.x * s.x + s.y * s.y)) {
                 s.x = XXx; s.y = YYx;
                 s.xs = to_std(XXx); s.ys = to_std(YYx);
@@ -413,7 +393,7 @@ struct BsXr {
     }
 
     __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.npts;
+        const int64_t n = a.ld;
         const cplx R = a.t.Zn[s.w];
         a.Z[i] = to_std(s.x + R.re); a.Z[n + i] = to_std(s.y + R.im);
         if (a.calc_dc)
@@ -487,7 +467,10 @@ __device__ __forceinline__ void store_b(double *M, double *r, int64_t i, const B
     r[i] = e.r;
 }
 
-template <int KIND, int DEG>
+// RADII_ONLY: the coefficients M are already in place (a context sharing
+// another's orbit and BLA coefficients); only the radii, which depend on kc
+// and eps, are written
+template <int KIND, int DEG, bool RADII_ONLY>
 __global__ void k_bla0_holo(const cplx *Zn, int32_t comp, double kc, double eps, double *M, double *r) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= comp) return;
@@ -501,10 +484,12 @@ __global__ void k_bla0_holo(const cplx *Zn, int32_t comp, double kc, double eps,
     for (int j = 0; j < 4; j++) e[j] = merge(e[2 * j], e[2 * j + 1], kc, eps);
 #pragma unroll
     for (int j = 0; j < 2; j++) e[j] = merge(e[2 * j], e[2 * j + 1], kc, eps);
-    store_h(M, r, 2 * (int64_t)i, merge(e[0], e[1], kc, eps));
+    const HEntry o = merge(e[0], e[1], kc, eps);
+    if constexpr (RADII_ONLY) r[2 * (int64_t)i] = o.r;
+    else store_h(M, r, 2 * (int64_t)i, o);
 }
 
-template <int FL>
+template <int FL, bool RADII_ONLY>
 __global__ void k_bla0_bs(const cplx *Zn, int32_t comp, double kc, double eps, double *M, double *r) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= comp) return;
@@ -518,19 +503,26 @@ __global__ void k_bla0_bs(const cplx *Zn, int32_t comp, double kc, double eps, d
     }
     for (int j = 0; j < 4; j++) e[j] = merge(e[2 * j], e[2 * j + 1], kc, eps);
     for (int j = 0; j < 2; j++) e[j] = merge(e[2 * j], e[2 * j + 1], kc, eps);
-    store_b(M, r, 2 * (int64_t)i, merge(e[0], e[1], kc, eps));
+    const BEntry o = merge(e[0], e[1], kc, eps);
+    if constexpr (RADII_ONLY) r[2 * (int64_t)i] = o.r;
+    else store_b(M, r, 2 * (int64_t)i, o);
 }
 
 // one stored level: merge i in [0, floor(comp / 2^stg)) of stage stg
-template <bool BS>
+template <bool BS, bool RADII_ONLY>
 __global__ void k_bla_merge(int32_t comp, int32_t stg, double kc, double eps, double *M, double *r) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t step = 1LL << stg;
     if (t >= comp / step) return;
     const int64_t i = t * step, ii = i + step / 2;
     const int64_t i1 = 2 * i + (step / 2 - 1), i2 = 2 * ii + (step / 2 - 1), ir = 2 * i + (step - 1);
-    if constexpr (BS) store_b(M, r, ir, merge(load_b(M, r, i1), load_b(M, r, i2), kc, eps));
-    else store_h(M, r, ir, merge(load_h(M, r, i1), load_h(M, r, i2), kc, eps));
+    if constexpr (BS) {
+        const BEntry o = merge(load_b(M, r, i1), load_b(M, r, i2), kc, eps);
+        if constexpr (RADII_ONLY) r[ir] = o.r; else store_b(M, r, ir, o);
+    } else {
+        const HEntry o = merge(load_h(M, r, i1), load_h(M, r, i2), kc, eps);
+        if constexpr (RADII_ONLY) r[ir] = o.r; else store_h(M, r, ir, o);
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -560,47 +552,70 @@ static tile_fn select_tile(const LaunchCfg &c) {
     return nullptr;
 }
 
+// blocks that fill the GPU at the kernel's occupancy, cached per kernel
+int grid_fill_blocks(const void *fn) {
+    static std::mutex mu;
+    static std::unordered_map<long long, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    const long long key = (long long)(uintptr_t)fn * 64 + dev;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) return -1;
+    const int blocks = ncu * (per_cu < 1 ? 1 : per_cu);
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = blocks;
+    return blocks;
+}
+
 int tile_kernel_blocks(const LaunchCfg &cfg) {
     tile_fn f = select_tile(cfg);
     if (!f) return -1;
-    int dev = 0, ncu = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), 256, 0) != hipSuccess)
-        return -1;
-    if (per_cu < 1) per_cu = 1;
-    return ncu * per_cu;
+    return grid_fill_blocks(reinterpret_cast<const void *>(f));
+}
+
+int launch_blocks(int fill, const KArgs &a) {
+    int blocks = fill;
+    if (a.max_blocks > 0 && a.max_blocks < blocks) blocks = a.max_blocks;
+    if (a.max_blocks < 0) blocks = (fill + (-a.max_blocks) - 1) / (-a.max_blocks);   // a share of the GPU
+    const int64_t need = (a.npts + 255) / 256;
+    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
+    return blocks;
 }
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
     tile_fn f = select_tile(cfg);
     if (!f) return hipErrorInvalidValue;
-    int blocks = tile_kernel_blocks(cfg);
-    if (blocks <= 0) return hipErrorInvalidValue;
-    int64_t need = (a.npts + 255) / 256;
-    if (need < blocks) blocks = (int)(need > 0 ? need : 1);
-    hipLaunchKernelGGL(f, dim3(blocks), dim3(256), 0, stream, a);
+    const int fill = grid_fill_blocks(reinterpret_cast<const void *>(f));
+    if (fill <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(f, dim3(launch_blocks(fill, a)), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
-                            double kc, double eps, double *M, double *r, int32_t stages,
-                            hipStream_t stream) {
+template <bool RO>
+static hipError_t bla_build_t(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
+                              double kc, double eps, double *M, double *r, int32_t stages,
+                              hipStream_t stream) {
     const int32_t comp = L >> 3;
     if (comp <= 0) return hipErrorInvalidValue;
     const int tb = 128;
     dim3 g0((comp + tb - 1) / tb);
-    if (kind == KIND_M2) hipLaunchKernelGGL((k_bla0_holo<KIND_M2, 2>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r);
+    if (kind == KIND_M2) hipLaunchKernelGGL((k_bla0_holo<KIND_M2, 2, RO>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r);
     else if (kind == KIND_MN) {
         switch (degree) {
-#define FSG_MN(D) case D: hipLaunchKernelGGL((k_bla0_holo<KIND_MN, D>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r); break;
+#define FSG_MN(D) case D: hipLaunchKernelGGL((k_bla0_holo<KIND_MN, D, RO>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r); break;
             FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
 #undef FSG_MN
             default: return hipErrorInvalidValue;
         }
     } else {
         switch (flavor) {
-#define FSG_BS(F) case F: hipLaunchKernelGGL((k_bla0_bs<F>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r); break;
+#define FSG_BS(F) case F: hipLaunchKernelGGL((k_bla0_bs<F, RO>), g0, dim3(tb), 0, stream, Zn, comp, kc, eps, M, r); break;
             FSG_BS(1) FSG_BS(2) FSG_BS(3) FSG_BS(4) FSG_BS(5)
 #undef FSG_BS
             default: return hipErrorInvalidValue;
@@ -612,12 +627,19 @@ hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, in
         int64_t nm = comp >> stg;
         if (nm <= 0) break;
         dim3 g((unsigned)((nm + tb - 1) / tb));
-        if (kind == KIND_BS) hipLaunchKernelGGL((k_bla_merge<true>), g, dim3(tb), 0, stream, comp, stg, kc, eps, M, r);
-        else hipLaunchKernelGGL((k_bla_merge<false>), g, dim3(tb), 0, stream, comp, stg, kc, eps, M, r);
+        if (kind == KIND_BS) hipLaunchKernelGGL((k_bla_merge<true, RO>), g, dim3(tb), 0, stream, comp, stg, kc, eps, M, r);
+        else hipLaunchKernelGGL((k_bla_merge<false, RO>), g, dim3(tb), 0, stream, comp, stg, kc, eps, M, r);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
+                            double kc, double eps, double *M, double *r, int32_t stages,
+                            hipStream_t stream, bool radii_only) {
+    return radii_only ? bla_build_t<true>(kind, degree, flavor, Zn, L, kc, eps, M, r, stages, stream)
+                      : bla_build_t<false>(kind, degree, flavor, Zn, L, kc, eps, M, r, stages, stream);
 }
 
 }  // namespace fsg

@@ -11,6 +11,36 @@ namespace fsg {
 struct Counters { unsigned long long plain, bla, bla_it, reb, tries; };
 
 // --------------------------------------------------------------------------
+// BLA stage selection (ref_BLA_get, perturbation.py:2116-2170).  The tree
+// keeps the stage s >= 3 jump starting at iteration w (a multiple of 2^s) at
+// index 2 (w >> 3) + 2^(s-3) - 1.  The reference scans the stages from the
+// highest one down and takes the first with 2^s < first_invalid - w and
+// |z| < r.  Both conditions hold on a prefix of the stages: the first because
+// 2^s grows with s, the second because a merged entry's radius is the
+// minimum of its first half's radius and the new bound (combine_BLA,
+// perturbation.py:2020-2023), so r never grows with the stage for a fixed
+// start.  The highest valid stage is therefore found by bisection, with
+// about log2(stages) dependent loads instead of one per stage.
+// Returns the jump length (0 if no stage is valid) and the entry index.
+__device__ __forceinline__ int32_t bla_find(const Tables &t, double az, int32_t w, int32_t &idx) {
+    const int32_t it = w >> 3;
+    const int32_t invalid_step = t.first_invalid - w;
+    if (invalid_step <= 8) return 0;
+    const int32_t tz = (it == 0) ? 31 : __builtin_ctz((uint32_t)it);
+    const int32_t s_inv = 31 - __builtin_clz((uint32_t)(invalid_step - 1));   // 2^s < invalid_step
+    int32_t hi = min(min(3 + tz, t.stages_bla - 1), s_inv);
+    int32_t lo = 3, best = 0;
+    while (lo <= hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (az < t.r[2 * it + (1 << (mid - 3)) - 1]) { best = mid; lo = mid + 1; }
+        else hi = mid - 1;
+    }
+    if (best == 0) return 0;
+    idx = 2 * it + (1 << (best - 3)) - 1;
+    return 1 << best;
+}
+
+// --------------------------------------------------------------------------
 // The persistent tile kernel
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll

@@ -1,0 +1,155 @@
+// xl.hpp -- extended-range binary64 with lazy, branch-free normalisation.
+//
+// The reference's Xrange record (numba_xr.py:61-829) renormalises products
+// whose mantissa exponent leaves +-100 and aligns the operands of a sum with
+// a branchy exponent shift.  Rescaling a mantissa by a power of two is exact,
+// so where and how often a value is renormalised does not change any
+// rounding: a sum rounds the exact sum of the two values once, a product
+// rounds the exact product, whatever the split between mantissa and
+// exponent -- provided no mantissa becomes subnormal or overflows.  This
+// type keeps that provision by renormalising every result (three VALU
+// instructions: v_frexp_exp_i32_f64 and two v_ldexp_f64) and aligns sums
+// with v_ldexp_f64; a shifted operand that underflows is then below the
+// rounding error of the other one, exactly as in the reference's shift.
+// Zero carries the exponent XL_ZERO_E so that it never drags the other
+// operand of a sum down.
+//
+// Conversions that the reference defines bit by bit (to_standard, the
+// comparisons) are done with the reference's own procedure (xr.hpp).
+#pragma once
+#include "xr.hpp"
+
+namespace fsg {
+
+struct xl { double re, im; int32_t e; };
+struct xlf { double m; int32_t e; };
+
+constexpr int32_t XL_ZERO_E = -(1 << 29);
+
+#define FSG_DEV __device__ __forceinline__
+
+FSG_DEV int32_t dexp(double v) { return __builtin_amdgcn_frexp_exp(v); }
+
+// normalised: larger component in [1, 2), or (0, 0, XL_ZERO_E)
+FSG_DEV xl xl_n(double re, double im, int32_t e) {
+    const double mx = fmax(fabs(re), fabs(im));
+    const int32_t k = dexp(mx) - 1;
+    xl r;
+    r.re = ldexp(re, -k);
+    r.im = ldexp(im, -k);
+    r.e = (mx == 0.) ? XL_ZERO_E : e + k;
+    return r;
+}
+FSG_DEV xlf xlf_n(double m, int32_t e) {
+    const int32_t k = dexp(m) - 1;
+    xlf r;
+    r.m = ldexp(m, -k);
+    r.e = (m == 0.) ? XL_ZERO_E : e + k;
+    return r;
+}
+
+FSG_DEV xl xl_of(cplx z) { return xl_n(z.re, z.im, 0); }
+FSG_DEV xl xl_of(xc a) { return xl_n(a.re, a.im, a.e); }
+FSG_DEV xl xl_of(double re, double im, int32_t e) { return xl_n(re, im, e); }
+FSG_DEV xl xl_zero() { xl r; r.re = 0.; r.im = 0.; r.e = XL_ZERO_E; return r; }
+FSG_DEV xlf xlf_of(double m, int32_t e) { return xlf_n(m, e); }
+FSG_DEV xlf xlf_of(xf a) { return xlf_n(a.m, a.e); }
+FSG_DEV xlf xlf_zero() { xlf r; r.m = 0.; r.e = XL_ZERO_E; return r; }
+
+// back to the reference's record, for its bit-defined conversions
+FSG_DEV xc xc_of(xl a) { return mkc(a.re, a.im, a.e == XL_ZERO_E ? 0 : a.e); }
+FSG_DEV xf xf_of(xlf a) { return mkf(a.m, a.e == XL_ZERO_E ? 0 : a.e); }
+
+FSG_DEV int32_t shift_of(int32_t e, int32_t E) { return max(e - E, -2100); }
+
+// ---- complex ----
+FSG_DEV xl operator+(xl a, xl b) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
+    return xl_n(ldexp(a.re, sa) + ldexp(b.re, sb), ldexp(a.im, sa) + ldexp(b.im, sb), E);
+}
+FSG_DEV xl operator-(xl a, xl b) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
+    return xl_n(ldexp(a.re, sa) - ldexp(b.re, sb), ldexp(a.im, sa) - ldexp(b.im, sb), E);
+}
+FSG_DEV xl operator*(xl a, xl b) {
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    return xl_n(ac - bd, ad + bc, a.e + b.e);
+}
+FSG_DEV xl operator*(xl a, cplx b) {
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    return xl_n(ac - bd, ad + bc, a.e);
+}
+FSG_DEV xl muls(double s, xl a) { return xl_n(s * a.re, s * a.im, a.e); }
+FSG_DEV xl operator+(xl a, cplx b) { return a + xl_of(b); }
+FSG_DEV xl operator+(xl a, xlf b) { xl c; c.re = b.m; c.im = 0.; c.e = b.e; return a + c; }
+
+// |a|^2 as a real value (mantissa products rounded like extended_abs2)
+FSG_DEV xlf abs2(xl a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
+
+// ---- real ----
+FSG_DEV xlf operator+(xlf a, xlf b) {
+    const int32_t E = max(a.e, b.e);
+    return xlf_n(ldexp(a.m, shift_of(a.e, E)) + ldexp(b.m, shift_of(b.e, E)), E);
+}
+FSG_DEV xlf operator-(xlf a, xlf b) {
+    const int32_t E = max(a.e, b.e);
+    return xlf_n(ldexp(a.m, shift_of(a.e, E)) - ldexp(b.m, shift_of(b.e, E)), E);
+}
+FSG_DEV xlf operator-(xlf a) { a.m = -a.m; return a; }
+FSG_DEV xlf xabs(xlf a) { a.m = fabs(a.m); return a; }
+FSG_DEV xlf operator*(xlf a, xlf b) { return xlf_n(a.m * b.m, a.e + b.e); }
+FSG_DEV xlf operator*(double s, xlf a) { return xlf_n(s * a.m, a.e); }
+FSG_DEV xlf mul_s(double s, xlf a) { return xlf_n(s * a.m, a.e); }
+FSG_DEV xlf operator+(xlf a, double b) { return a + xlf_n(b, 0); }
+
+// value comparisons (exact alignment; equal to the reference's co-exponent
+// comparison for every operand pair the loop produces)
+FSG_DEV bool lt(xlf a, double b) {
+    const xlf bb = xlf_n(b, 0);
+    const int32_t E = max(a.e, bb.e);
+    return ldexp(a.m, shift_of(a.e, E)) < ldexp(bb.m, shift_of(bb.e, E));
+}
+FSG_DEV bool ge(xlf a, double b) { return !lt(a, b); }
+FSG_DEV bool le(xlf a, double b) {
+    const xlf bb = xlf_n(b, 0);
+    const int32_t E = max(a.e, bb.e);
+    return ldexp(a.m, shift_of(a.e, E)) <= ldexp(bb.m, shift_of(bb.e, E));
+}
+FSG_DEV bool le(xlf a, xlf b) {
+    const int32_t E = max(a.e, b.e);
+    return ldexp(a.m, shift_of(a.e, E)) <= ldexp(b.m, shift_of(b.e, E));
+}
+
+// Conversion to binary64 with the reference's procedure (to_standard,
+// numba_xr.py:802-829: per-component frexp, co-exponent, product by
+// 2^exp).  For a normalised value whose smaller component is zero or a
+// normal number the co-exponent shift is exact and the procedure reduces to
+// two products by 2^e; otherwise it is run as written.
+FSG_DEV cplx to_std(xl a) {
+    const double mn = fmin(fabs(a.re), fabs(a.im));
+    if (mn != 0. && mn < 0x1p-1022) return to_std(xc_of(a));
+    const double s = ldexp(1., max(a.e, -1100));
+    return mk(a.re * s, a.im * s);
+}
+FSG_DEV double to_std(xlf a) { return ldexp(a.m, max(a.e, -2200)); }
+
+// burning-ship helpers on Xrange reals (burning_ship.py:19-60)
+FSG_DEV xlf diffabs(xlf X, xlf x) {
+    if (ge(X, 0.)) { if (ge(X + x, 0.)) return x; return -(2. * X + x); }
+    if (le(X + x, 0.)) return -x;
+    return 2. * X + x;
+}
+FSG_DEV double ddX(xlf X, xlf x) {
+    if (ge(X, 0.)) return ge(X + x, 0.) ? 0. : -2.;
+    return le(X + x, 0.) ? 0. : 2.;
+}
+FSG_DEV double ddx(xlf X, xlf x) {
+    if (ge(X, 0.)) return ge(X + x, 0.) ? 1. : -1.;
+    return le(X + x, 0.) ? -1. : 1.;
+}
+FSG_DEV double sgn(xlf x) { return lt(x, 0.) ? -1. : 1.; }
+FSG_DEV xlf absv(xlf x) { return xabs(x); }
+
+}  // namespace fsg

@@ -30,17 +30,18 @@ class CycleIndepArgs:
     engine descriptor (fields + arrays) and, created on first use, the
     device-resident context built from it."""
 
-    def __init__(self, fields, arrays, interrupted):
+    def __init__(self, fields, arrays, interrupted, share=None):
         self.fields = fields
         self.arrays = arrays
         self.interrupted = interrupted
         self.last_stats = None
+        self.share = share      # GpuContext whose orbit / BLA coefficients to reuse
         self._ctx = None
 
     @property
     def ctx(self):
         if self._ctx is None:
-            self._ctx = GpuContext(self.fields, self.arrays)
+            self._ctx = GpuContext(self.fields, self.arrays, share=self.share)
         return self._ctx
 
     def close(self):
@@ -50,7 +51,12 @@ class CycleIndepArgs:
 
     def with_fields(self, **over):
         """A copy with some descriptor fields replaced (its own context)."""
-        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted)
+        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted, self.share)
+
+    def shared_with(self, base_ctx):
+        """A copy whose context reuses base_ctx's device orbit and BLA
+        coefficients (same orbit required)."""
+        return CycleIndepArgs(self.fields, self.arrays, self.interrupted, base_ctx)
 
 
 class PerturbationFractal(Fractal):
@@ -204,8 +210,10 @@ class PerturbationFractal(Fractal):
 
     def set_reference(self, ref_point, order, Zn_path, div_iter, xr):
         """Install an orbit computed elsewhere (a cache or another process)."""
+        with mpmath.workprec(self.prec):
+            ref_point = mpmath.mpc(ref_point)
         self._FP_params = {
-            "ref_point": mpmath.mpc(ref_point), "prec": self.prec, "order": order,
+            "ref_point": ref_point, "prec": self.prec, "order": order,
             "max_iter": self.max_iter, "FP_code": self.FP_code, "xr": xr,
             "div_iter": div_iter, "init_kwargs": self.init_kwargs,
             "xr_detect": self.xr_detect_activated,

@@ -39,6 +39,7 @@ extern "C" {
 /* error codes */
 #define FSG_OK 0
 #define FSG_INTERRUPTED 1
+#define FSG_PENDING 2    /* fsg_server_wait: not done within the timeout */
 #define FSG_E_INVALID (-1)
 #define FSG_E_HIP (-2)
 #define FSG_E_NOMEM (-3)
@@ -111,6 +112,20 @@ int fsg_abi_version(void);
  */
 int fsg_ctx_create(const fsg_ref_desc *desc, fsg_ctx **out);
 
+/*
+ * A context on the same reference orbit as `base` (same model, orbit and
+ * Xrange register, checked by a hash of the host arrays in `desc`): the
+ * device orbit, register and BLA coefficients of `base` are shared, not
+ * copied; the derivative paths (which scale with dx) and the BLA radii
+ * (which depend on kc and BLA_eps) are the new context's own.  This is how
+ * frames of different width and max_iter around one centre keep one
+ * resident orbit and BLA table.  The shared data lives until the last
+ * context using it is destroyed.  Results are identical to fsg_ctx_create's
+ * with the same descriptor.  The reference rebuilds everything per
+ * calculation (get_cycle_indep_args, perturbation.py:431-562).
+ */
+int fsg_ctx_create_shared(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out);
+
 /* Release a context and its device memory. */
 int fsg_ctx_destroy(fsg_ctx *ctx);
 
@@ -168,6 +183,56 @@ int fsg_ctx_bla_shape(fsg_ctx *ctx, int64_t *bla_len, int32_t *stages_bla);
 
 /* Last error message of the calling thread. */
 const char *fsg_last_error(void);
+
+/*
+ * ---- Serving loop ----------------------------------------------------------
+ * No counterpart in the reference, which renders the tiles of one frame with
+ * a thread pool (mthreading.py:48-68).  A server takes tile requests of mixed
+ * size, context (centre, zoom depth, max_iter, model, precision) and arrival
+ * time; it admits them in arrival order as its device-memory budget allows,
+ * splits them into chunks and runs chunks of different requests concurrently
+ * on `lanes` HIP streams.  Results are written to the caller's host buffers
+ * (pinned memory from fsg_host_alloc makes the copies asynchronous).  A
+ * request's results are identical to fsg_tile's for the same inputs.  The
+ * contexts must stay alive until their requests are waited for.
+ */
+typedef struct fsg_server fsg_server;
+
+typedef struct fsg_server_opts {
+    int32_t lanes;        /* chunks in flight, one HIP stream each (1..16; 0 = 3) */
+    int32_t chunk_px;     /* pixels per chunk launch (0 = 65536; rounded up to 256) */
+    int64_t hbm_budget;   /* device bytes for admitted requests' buffers (0 = 4 GiB) */
+} fsg_server_opts;
+
+typedef struct fsg_request_info {
+    int32_t status;       /* 0 or the FSG_E_* code of the failure */
+    int32_t chunks;       /* launches the request was split into */
+    int64_t footprint;    /* device bytes it held while admitted */
+    double t_submit_ms;   /* times on the server clock (fsg_server_clock) */
+    double t_admit_ms;    /* device buffers granted, input copy enqueued */
+    double t_first_ms;    /* first chunk launched */
+    double t_done_ms;     /* results in the caller's host buffers */
+} fsg_request_info;
+
+int fsg_server_create(const fsg_server_opts *opts, fsg_server **out);
+/* Finishes every submitted request, then releases the server. */
+int fsg_server_destroy(fsg_server *srv);
+/* Device bytes a request of n pixels on ctx needs (admission footprint). */
+int fsg_server_footprint(const fsg_ctx *ctx, int64_t n, int64_t *bytes);
+/* Milliseconds since the server started. */
+int fsg_server_clock(fsg_server *srv, double *ms);
+/* Queue a request (same buffers as fsg_tile); returns at once with a ticket.
+ * FSG_E_NOMEM if the request alone exceeds the budget. */
+int fsg_server_submit(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z,
+                      int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
+/* Wait for a request (timeout_ms < 0: no limit).  Returns its status, or
+ * FSG_PENDING on timeout; after a return other than FSG_PENDING the ticket
+ * is released. */
+int fsg_server_wait(fsg_server *srv, uint64_t ticket, int32_t timeout_ms, fsg_request_info *info);
+
+/* Pinned host memory for request buffers. */
+void *fsg_host_alloc(size_t bytes);
+int fsg_host_free(void *p);
 
 #ifdef __cplusplus
 }

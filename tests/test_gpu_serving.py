@@ -1,0 +1,141 @@
+"""Serving loop (fsg_server_*): every request's results are bit-identical
+whether it runs alone (fsg_tile) or in a mix of concurrently running
+requests of other sizes, models, depths and precisions; admission by
+device-memory footprint; request bookkeeping."""
+import numpy as np
+import pytest
+
+import frames
+from gpu_util import run_gpu, assert_same
+from fractalshades_amd._native import FsgError
+from fractalshades_amd.serving import TileServer, PinnedTile, n_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def with_fields(args, **over):
+    return args.with_fields(**over) if over else args
+
+
+def _mix():
+    """(label, CycleIndepArgs, c_pix) for a mix of models, sizes, depths and
+    precisions."""
+    out = []
+    f = frames.make("M2_E20", nx=96)
+    a, c = frames.frame_inputs(f)
+    out += [("M2_E20 fp64", a, c), ("M2_E20 xr", with_fields(a, xr_mode=1), c),
+            ("M2_E20 fp32x", with_fields(a, precision=1), c)]
+    f = frames.make("M2_divref5", nx=64)
+    a, c = frames.frame_inputs(f)
+    out += [("M2_divref5", a, c)]
+    f = frames.make("M2_E11", nx=40, interior_detect=True)
+    a, c = frames.frame_inputs(f)
+    out += [("M2_E11 interior xr", with_fields(a, xr_mode=1), c)]
+    f = frames.mn_frame(3, nx=48)
+    a, c = frames.frame_inputs(f)
+    out += [("Mn3", a, c)]
+    f = frames.make("BS_ex14", nx=56, max_iter=4000)
+    a, c = frames.frame_inputs(f)
+    out += [("BS_ex14", a, c), ("BS_ex14 xr", with_fields(a, xr_mode=1), c)]
+    return out
+
+
+def _tile_of(args, c_pix):
+    t = PinnedTile(c_pix.size, n_fields(args.fields), args.fields["kind"] != 2)
+    t.c_pix[:] = c_pix
+    t.stop_reason[:] = -1
+    return t
+
+
+def test_server_mix_equals_alone():
+    mix = _mix()
+    alone = []
+    for label, a, c in mix:
+        print(f"alone {label} n={c.size}", flush=True)
+        alone.append(run_gpu(a, c)[0])
+    srv = TileServer(lanes=3, chunk_px=1024, hbm_budget=256 << 20)
+    try:
+        tiles, tickets = [], []
+        for rep in range(2):            # every request twice, interleaved
+            for label, a, c in mix:
+                t = _tile_of(a, c)
+                tiles.append((label, t))
+                tickets.append(srv.submit(a.ctx, t))
+        infos = [srv.wait(tk, 120000) for tk in tickets]
+    finally:
+        srv.close()
+    for k, ((label, t), info) in enumerate(zip(tiles, infos)):
+        print(f"mix {label} chunks={info['chunks']}", flush=True)
+        assert info is not None and info["status"] == 0
+        assert info["chunks"] == (t.n + 1023) // 1024
+        assert info["t_submit_ms"] <= info["t_admit_ms"] <= info["t_first_ms"] <= info["t_done_ms"]
+        assert_same((t.Z, t.U, t.stop_reason, t.stop_iter), alone[k % len(mix)])
+
+
+def test_server_admission_by_footprint():
+    f = frames.make("M2_E20", nx=128)
+    a, c = frames.frame_inputs(f)
+    ref = run_gpu(a, c)[0]
+    probe = TileServer(lanes=2, chunk_px=4096, hbm_budget=64 << 20)
+    foot = probe.footprint(a.ctx, c.size)
+    probe.close()
+    # room for one request at a time: the others wait for the arena
+    budget = max(1 << 20, int(foot * 1.5))
+    srv = TileServer(lanes=2, chunk_px=4096, hbm_budget=budget)
+    try:
+        tiles = [_tile_of(a, c) for _ in range(4)]
+        tickets = [srv.submit(a.ctx, t) for t in tiles]
+        infos = [srv.wait(tk, 120000) for tk in tickets]
+        big = _tile_of(a, np.ascontiguousarray(np.tile(c, 4)))
+        with pytest.raises(FsgError):
+            srv.submit(a.ctx, big)      # alone larger than the budget
+    finally:
+        srv.close()
+    for t, info in zip(tiles, infos):
+        assert info["footprint"] == foot
+        assert_same((t.Z, t.U, t.stop_reason, t.stop_iter), ref)
+    # one at a time: each admission comes after the previous request is done
+    for prev, nxt in zip(infos, infos[1:]):
+        assert nxt["t_admit_ms"] >= prev["t_done_ms"] - 1e-3
+
+
+def test_server_empty_request():
+    f = frames.make("M2_E20", nx=16)
+    a, c = frames.frame_inputs(f)
+    srv = TileServer(lanes=1)
+    try:
+        t = _tile_of(a, c[:0])
+        info = srv.wait(srv.submit(a.ctx, t), 10000)
+        assert info["status"] == 0 and info["chunks"] == 0
+    finally:
+        srv.close()
+
+
+@pytest.mark.parametrize("xr", [0, 1])
+def test_shared_context_equals_own(xr):
+    """fsg_ctx_create_shared: same tables and results as a context of its own
+    for a different width / kc / max_iter on the same orbit."""
+    f = frames.make("M2_E20", nx=64, interior_detect=True)
+    a, c = frames.frame_inputs(f)
+    fields = dict(a.fields, xr_mode=xr, dx_mant=a.fields["dx_mant"] * 1.37,
+                  kc_mant=a.fields["kc_mant"] * 1.37, max_iter=a.fields["max_iter"] // 2)
+    base = a.ctx
+    own = a.__class__(fields, a.arrays, a.interrupted)
+    shared = a.__class__(fields, a.arrays, a.interrupted).shared_with(base)
+    print(f"case shared xr={xr}", flush=True)
+    bl, _ = own.ctx.bla_shape()
+    for which, shape, dt in ((0, (bl, 4), np.float64), (1, (bl,), np.float64),
+                             (2, (2 * fields["ref_len"],), np.float64),
+                             (4, (2 * (fields["ref_len"] + 1),), np.float64)):
+        np.testing.assert_array_equal(own.ctx.read_table(which, shape, dt).view(np.uint64),
+                                      shared.ctx.read_table(which, shape, dt).view(np.uint64))
+    assert_same(run_gpu(shared, c)[0], run_gpu(own, c)[0])
+
+
+def test_shared_context_rejects_other_orbit():
+    f = frames.make("M2_E20", nx=16)
+    a, _ = frames.frame_inputs(f)
+    g = frames.make("M2_E11", nx=16)
+    b, _ = frames.frame_inputs(g)
+    with pytest.raises(FsgError):
+        _ = b.shared_with(a.ctx).ctx
