@@ -91,6 +91,64 @@ def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
     }
 
 
+def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
+    """The request stream: seeded requests of mixed size / width / max_iter
+    on eight centres (fractalshades_amd.serving), first all at once (the
+    makespan gives the capacity), then with Poisson arrivals at `rate_frac`
+    of that capacity (latency under sustained load)."""
+    from fractalshades_amd import serving
+    t0 = time.perf_counter()
+    centres = serving.serve_centres()
+    specs = serving.serve_requests(n_requests, len(centres), seed=seed)
+    bases = {}
+    reqs = []
+    for sp in specs:
+        f, cia, c_pix = serving.build_request(sp, centres, precision)
+        key = (sp.centre, cia.fields["xr_mode"])
+        if key not in bases:
+            bases[key] = cia.ctx            # resident orbit + BLA coefficients of this centre
+        else:
+            cia = cia.shared_with(bases[key])
+        _ = cia.ctx
+        reqs.append((cia, c_pix))
+    prep_s = time.perf_counter() - t0
+    tiles = serving.pinned_tiles(reqs)
+    srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
+    try:
+        burst = serving.run_stream(srv, reqs, None, tiles=tiles)
+        t_first = min(r["t_submit_ms"] for r in burst)
+        makespan_s = (max(r["t_done_ms"] for r in burst) - t_first) * 1e-3
+        rate = rate_frac * n_requests / makespan_s
+        steady = serving.run_stream(srv, reqs, rate, seed=seed + 1, tiles=tiles)
+    finally:
+        srv.close()
+
+    def lat(rs, size=None):
+        v = np.array([r["latency_ms"] for r in rs if size is None or r["pixels"] == size])
+        if v.size == 0:
+            return None
+        return {"n": int(v.size), "p50_ms": float(np.percentile(v, 50)),
+                "p99_ms": float(np.percentile(v, 99)), "max_ms": float(v.max())}
+
+    pix_iters = sum(r["pixel_iters"] for r in burst)
+    span = (max(r["t_done_ms"] for r in steady) - min(r["t_submit_ms"] for r in steady)) * 1e-3
+    out = {
+        "requests": n_requests, "precision": precision, "host_prep_s": round(prep_s, 1),
+        "pixels": int(sum(r["pixels"] for r in burst)), "pixel_iters": int(pix_iters),
+        "burst": {"makespan_s": makespan_s, "pixel_iters_per_s": pix_iters / makespan_s,
+                  "latency": lat(burst)},
+        "poisson": {"rate_req_per_s": rate, "span_s": span,
+                    "pixel_iters_per_s": sum(r["pixel_iters"] for r in steady) / span,
+                    "latency": lat(steady), "latency_512sq": lat(steady, 512 * 512),
+                    "latency_by_size": {f"{nx}x{ny}": lat(steady, nx * ny) for nx, ny in serving.SIZES}},
+    }
+    for t in tiles:
+        t.close()
+    for cia, _ in reqs:
+        cia.close()
+    return out
+
+
 class Leg:
     """One workload x precision on the GPU: device buffers + timed steps."""
 
@@ -105,12 +163,9 @@ class Leg:
         c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
         self.c_pix = c_pix
         n = self.n = c_pix.size
-        if fields["kind"] == 2:
-            self.nz = 2 + (4 if fields["calc_dzndc"] else 0)
-            z_words = self.nz * n
-        else:
-            self.nz = 1 + int(bool(fields["calc_dzndc"])) + int(bool(fields["interior_detect"]))
-            z_words = 2 * self.nz * n
+        from fractalshades_amd._native import n_z_fields
+        self.nz = n_z_fields(fields)
+        z_words = self.nz * n * (1 if fields["kind"] == 2 else 2)
         self.d_pix = torch.from_numpy(c_pix.view(np.float64)).to(dev)
         self.d_Z = torch.empty(z_words, dtype=torch.float64, device=dev)
         self.d_U = torch.empty(n, dtype=torch.int32, device=dev)
@@ -175,6 +230,7 @@ def main():
     ap.add_argument("--deep", default="C2_2000", help="second frame (1e-2000); '' to skip")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
+    ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (PMC counter calibration)")
     args = ap.parse_args()
@@ -251,6 +307,14 @@ def main():
             "flops_per_unit": {"plain": fp, "bla": fb, "lookup": fl, "rebase": fr},
         },
     }
+    if args.serve > 0:
+        serve = {"fp64": serving_leg(args.serve, "fp64")}
+        if not args.no_fp32x:
+            serve["fp32x"] = serving_leg(args.serve, "fp32x")
+        out["config"]["serving"] = serve
+        lat512 = serve["fp64"]["poisson"]["latency_512sq"]
+        if lat512:
+            out["tile_latency_512sq_ms"] = {"p50": lat512["p50_ms"], "p99": lat512["p99_ms"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
     print(json.dumps(out), flush=True)

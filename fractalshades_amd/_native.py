@@ -27,6 +27,16 @@ EXPORTS = (
 FSG_PENDING = 2
 
 
+def n_z_fields(fields):
+    """Rows of the Z result array of a descriptor (the reference's
+    complex_codes): holomorphic zn, [dzndz], [dzndc], [zn_orbit]; burning
+    ship xn, yn, [4 Jacobian], [xn_orbit, yn_orbit]."""
+    orb = int(bool(fields.get("calc_orbit", 0)))
+    if fields["kind"] == FSG_KIND_BS:
+        return 2 + (4 if fields["calc_dzndc"] else 0) + 2 * orb
+    return 1 + int(bool(fields["calc_dzndc"])) + int(bool(fields["interior_detect"])) + orb
+
+
 class FsgError(RuntimeError):
     """Error raised by the native engine (or its absence)."""
 
@@ -50,7 +60,8 @@ class RefDesc(ctypes.Structure):
         ("max_iter", ctypes.c_int64),
         ("M_divergence", ctypes.c_double), ("epsilon_stationnary", ctypes.c_double),
         ("calc_dzndc", ctypes.c_int32), ("interior_detect", ctypes.c_int32),
-        ("precision", ctypes.c_int32), ("pad2", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("calc_orbit", ctypes.c_int32),
+        ("backshift", ctypes.c_int32), ("pad2", ctypes.c_int32),
     ]
 
 
@@ -129,7 +140,7 @@ def load(path=None):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.fsg_abi_version() != 1:
+        if lib.fsg_abi_version() != 2:
             raise FsgError("libfsgpu.so ABI version mismatch")
         if path is None:
             _lib = lib

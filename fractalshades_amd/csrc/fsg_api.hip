@@ -284,6 +284,10 @@ int validate(const fsg_ref_desc *d) {
     if (d->ref_order < 1) return fail(FSG_E_INVALID, "ref_order must be >= 1");
     if (d->precision != FSG_PREC_FP64 && d->precision != FSG_PREC_FP32X)
         return fail(FSG_E_INVALID, "unknown precision mode");
+    if (d->calc_orbit && (d->backshift < 1 || d->backshift > (1 << 20)))
+        return fail(FSG_E_INVALID, "calc_orbit needs backshift in 1..2^20");
+    if (d->calc_orbit && d->precision != FSG_PREC_FP64)
+        return fail(FSG_E_UNSUPPORTED, "calc_orbit is computed by the FP64 kernels only");
     if (d->bla_on && stages_of(d->ref_len) <= 3)
         return fail(FSG_E_UNSUPPORTED, "BLA needs a reference orbit longer than 8");
     int64_t valid = std::min(d->ref_len, d->ref_div_iter);
@@ -312,8 +316,10 @@ int fill_args(fsg_ctx *c, KArgs &a) {
     a.calc_dz = (c->d.kind != FSG_KIND_BS && c->d.interior_detect) ? 1 : 0;
     a.M2 = c->d.M_divergence * c->d.M_divergence;
     a.eps2 = c->d.epsilon_stationnary * c->d.epsilon_stationnary;
-    if (c->d.kind == FSG_KIND_BS) a.nz = 2 + (a.calc_dc ? 4 : 0);
-    else a.nz = 1 + a.calc_dc + a.calc_dz;
+    a.calc_orbit = c->d.calc_orbit ? 1 : 0;
+    a.backshift = c->d.calc_orbit ? c->d.backshift : 1;
+    if (c->d.kind == FSG_KIND_BS) a.nz = 2 + (a.calc_dc ? 4 : 0) + 2 * a.calc_orbit;
+    else a.nz = 1 + a.calc_dc + a.calc_dz + a.calc_orbit;
     return 0;
 }
 
@@ -321,12 +327,18 @@ LaunchCfg cfg_of(const fsg_ctx *c) {
     LaunchCfg g;
     g.kind = c->d.kind; g.degree = c->d.degree; g.flavor = c->d.flavor; g.xr = c->d.xr_mode ? 1 : 0;
     g.precision = c->d.precision;
+    g.orbit = c->d.calc_orbit ? 1 : 0;
     return g;
 }
 
+int n_fields(const fsg_ctx *c) {
+    const int orb = c->d.calc_orbit ? 1 : 0;
+    return (c->d.kind == FSG_KIND_BS) ? 2 + (c->d.calc_dzndc ? 4 : 0) + 2 * orb
+                                      : 1 + (c->d.calc_dzndc ? 1 : 0) + (c->d.interior_detect ? 1 : 0) + orb;
+}
+
 size_t z_bytes(const fsg_ctx *c, int64_t n) {
-    int nz = (c->d.kind == FSG_KIND_BS) ? 2 + (c->d.calc_dzndc ? 4 : 0)
-                                        : 1 + (c->d.calc_dzndc ? 1 : 0) + (c->d.interior_detect ? 1 : 0);
+    int nz = n_fields(c);
     return (size_t)nz * (size_t)n * (c->d.kind == FSG_KIND_BS ? 8 : 16);
 }
 
@@ -354,10 +366,7 @@ int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8
 namespace fsg {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 int ctx_holomorphic(const fsg_ctx *c) { return c->d.kind != FSG_KIND_BS; }
-int ctx_nz(const fsg_ctx *c) {
-    return c->d.kind == FSG_KIND_BS ? 2 + (c->d.calc_dzndc ? 4 : 0)
-                                    : 1 + (c->d.calc_dzndc ? 1 : 0) + (c->d.interior_detect ? 1 : 0);
-}
+int ctx_nz(const fsg_ctx *c) { return n_fields(c); }
 size_t ctx_z_bytes(const fsg_ctx *c, int64_t n) { return z_bytes(c, n); }
 // pixels [0, n) of pix into columns [0, n) of outputs with row stride ld
 int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, double *Z, int32_t *U,

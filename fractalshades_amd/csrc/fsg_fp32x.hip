@@ -47,6 +47,18 @@ __device__ __forceinline__ xs muls(double s, xs a) {
 }
 __device__ __forceinline__ xs xs_zero() { return xs{0.f, 0.f, XS_ZERO_E}; }
 
+// sum of two unnormalised terms m1 2^e1 + m2 2^e2, normalised
+__device__ __forceinline__ xs xs_sum(float x1, float y1, int32_t e1, float x2, float y2, int32_t e2) {
+    const int32_t E = max(e1, e2);
+    const int32_t s1 = max(e1 - E, -200), s2 = max(e2 - E, -200);
+    return xs_norm(ldexpf(x1, s1) + ldexpf(x2, s2), ldexpf(y1, s1) + ldexpf(y2, s2), E);
+}
+// mantissa product
+__device__ __forceinline__ void cmul(float ax, float ay, float bx, float by, float &rx, float &ry) {
+    rx = ax * bx - ay * by;
+    ry = ax * by + ay * bx;
+}
+
 // binary64 value <-> xs
 __device__ __forceinline__ xs xs_from(double re, double im, int32_t e) {
     double mx = fmax(fabs(re), fabs(im));
@@ -158,9 +170,39 @@ struct HoloF32 {
         }
         s.n += 1; k.plain++;
         const xs Zr = Zx[s.w];
-        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xs_zero() : DC[s.w]);
-        if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ[0] : DZ[s.w]);
-        s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+        bool fast = false;
+        if constexpr (KIND == KIND_M2) fast = Zr.e >= -60;
+        if (fast) {
+            // z^2 + c with one shared exponent per quantity: the orbit point
+            // and z's own value (u, flushed to 0 once |z| < 2^-126, where
+            // its square is below binary32 resolution next to 2 Z z) are
+            // plain floats, so each product is 4 flops and each quantity is
+            // renormalised once per step
+            const float Zf_x = ldexpf(Zr.x, Zr.e), Zf_y = ldexpf(Zr.y, Zr.e);
+            const float ux = ldexpf(s.z.x, s.z.e), uy = ldexpf(s.z.y, s.z.e);
+            const float sx = Zf_x + ux, sy = Zf_y + uy;               // Z + z
+            float px, py;
+            if (a.calc_dc) {                                          // 2 ((Z + z) d + Zd z)
+                float qx, qy;
+                cmul(s.d.x, s.d.y, sx, sy, px, py);
+                const xs Zd = s.dyn ? xs_zero() : DC[s.w];
+                cmul(Zd.x, Zd.y, s.z.x, s.z.y, qx, qy);
+                s.d = xs_sum(px, py, s.d.e + 1, qx, qy, Zd.e + s.z.e + 1);
+            }
+            if (a.calc_dz) {                                          // 2 ((Z + z) h + Zh z)
+                float qx, qy;
+                cmul(s.h.x, s.h.y, sx, sy, px, py);
+                const xs Zh = s.nul ? DZ[0] : DZ[s.w];
+                cmul(Zh.x, Zh.y, s.z.x, s.z.y, qx, qy);
+                s.h = xs_sum(px, py, s.h.e + 1, qx, qy, Zh.e + s.z.e + 1);
+            }
+            cmul(s.z.x, s.z.y, sx + Zf_x, sy + Zf_y, px, py);         // z (z + 2 Z)
+            s.z = xs_sum(px, py, s.z.e, s.c.x, s.c.y, s.c.e);
+        } else {
+            if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xs_zero() : DC[s.w]);
+            if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ[0] : DZ[s.w]);
+            s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+        }
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;

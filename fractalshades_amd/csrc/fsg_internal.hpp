@@ -53,14 +53,18 @@ struct KArgs {
     double *Z; int32_t *U; int8_t *stop_reason; int32_t *stop_iter;
     int64_t ld;
     int32_t nz; int32_t max_blocks;   // grid cap: 0 fill the GPU, > 0 blocks, -k 1/k of the GPU
+    int32_t calc_orbit, backshift;    // orbit point at exit - backshift in the last field(s)
     // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
     // rebase, bla lookups)
     unsigned long long *work;
 };
 
-struct LaunchCfg { int kind, degree, flavor, xr, precision; };
+struct LaunchCfg { int kind, degree, flavor, xr, precision, orbit; };
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
+// the calc_orbit instantiations (fsg_orbit.hip)
+typedef void (*tile_fn)(const KArgs);
+tile_fn select_tile_orbit(const LaunchCfg &cfg);
 // radii_only: M already holds the coefficients (shared with another
 // context of the same orbit); write the radii r only
 hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,

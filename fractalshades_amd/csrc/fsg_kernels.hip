@@ -14,393 +14,12 @@
 //   model kernels: models/mandelbrot_M2.py:599-622, mandelbrot_Mn.py:646-729,
 //                  burning_ship.py:12-60, 440-859
 //   BLA: numba_make_BLA* / combine_BLA* / ref_BLA_get, perturbation.py:1819-2170
-#include "models.hpp"
-#include "tile_loop.hpp"
-#include "xl.hpp"
+#include "tile_models.hpp"
 
 #include <mutex>
 #include <unordered_map>
 
 namespace fsg {
-
-// BLA lookup on binary64 |z| (ref_BLA_get, perturbation.py:2116-2170)
-__device__ __forceinline__ int32_t bla_lookup(const Tables &t, double zre, double zim,
-                                              int32_t w, int32_t &idx) {
-    return bla_find(t, hypot2(zre, zim), w, idx);
-}
-
-// ---- pixel -> c (ref_path_c_from_pix + lin_proj_impl, perturbation.py:2214-2230, 2665-2672)
-FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
-    double x1 = a.lin[0] * px + a.lin[1] * py;
-    double y1 = a.lin[2] * px + a.lin[3] * py;
-    xc cx = renorm_c(a.dx_m * x1, a.dx_m * y1, a.dx_e);
-    return cx + mkc(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
-}
-
-FSG_HD cplx zc() { return mk(0., 0.); }
-FSG_HD xc zx() { return mkc(0., 0., 0); }
-
-__device__ __forceinline__ xl c_from_pix_xl(const KArgs &a, double px, double py) {
-    const double x1 = a.lin[0] * px + a.lin[1] * py;
-    const double y1 = a.lin[2] * px + a.lin[3] * py;
-    return xl_of(a.dx_m * x1, a.dx_m * y1, a.dx_e) + xl_of(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
-}
-
-// --------------------------------------------------------------------------
-// Holomorphic models, binary64 deltas
-template <int KIND, int DEG>
-struct HoloStd {
-    struct State { cplx z, d, h, c; int32_t w, n; bool dyn, nul; };
-
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        s.c = to_std(c_from_pix(a, a.pix[2 * i], a.pix[2 * i + 1]));
-        s.z = zc(); s.d = zc(); s.h = zc();
-        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
-    }
-
-    __device__ static int step(State &s, const KArgs &a, Counters &k) {
-        const Tables &t = a.t;
-        while (t.bla_on && (s.w & 7) == 0) {
-            int32_t idx;
-            int32_t st = bla_lookup(t, s.z.re, s.z.im, s.w, idx);
-            k.tries++;
-            if (st == 0) break;
-            {
-                const double *Mb = t.M + 4 * idx;
-                cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
-                s.n += st;
-                s.w += st;                      // w + st < first_invalid <= order
-                s.z = A * s.z + B * s.c;
-                if (a.calc_dc) s.d = A * s.d;
-                if (a.calc_dz) s.h = A * s.h;
-                k.bla++; k.bla_it += st;
-            }
-        }
-        s.n += 1; k.plain++;
-        const cplx Zr = t.Zn[s.w];
-        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zc() : t.dc[s.w]);
-        if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? t.dz[0] : t.dz[s.w]);
-        s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        if (s.n >= a.max_iter) return 0;
-        if (a.calc_dz) {
-            cplx nx = s.nul ? t.dz[0] : (s.n == t.order ? t.dz[t.L] : t.dz[s.w]);
-            cplx q = s.h + nx;
-            if (q.re * q.re + q.im * q.im < a.eps2) return 2;
-        }
-        const cplx ZZ = s.z + t.Zn[s.w];
-        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
-        bool reb = (s.w >= t.div_iter - 1);
-        if (!reb) {
-            s.dyn = (fabs(ZZ.re) <= fabs(s.z.re)) && (fabs(ZZ.im) <= fabs(s.z.im));
-            reb = s.dyn;
-        }
-        if (reb) {
-            s.z = ZZ;
-            if (a.calc_dc) s.d = s.d + t.dc[s.w];
-            if (a.calc_dz) {
-                if (!s.nul) s.h = s.h + (s.n == t.order ? t.dz[t.L] : t.dz[s.w]);
-                s.nul = true;
-            }
-            s.w = 0;
-            k.reb++;
-        }
-        return -1;
-    }
-
-    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.ld;
-        cplx z = s.z + a.t.Zn[s.w];
-        a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
-        int f = 1;
-        if (a.calc_dz) { a.Z[2 * (n + i)] = s.h.re; a.Z[2 * (n + i) + 1] = s.h.im; f = 2; }
-        if (a.calc_dc) {
-            cplx d = s.d + a.t.dc[s.w];
-            a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
-        }
-        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
-    }
-};
-
-// Holomorphic models, Xrange deltas (dx < 1e-300).  The deltas are kept in
-// the lazily normalised extended-range form of xl.hpp: every result is
-// renormalised without branches, which rounds exactly like the reference's
-// Xrange record (a rescaling by a power of two is exact).
-template <int KIND, int DEG>
-struct HoloXr {
-    struct State { xl z, d, h, c; cplx zs; int32_t w, n; bool dyn, nul; };
-
-    __device__ static xl DC(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xl_of(m.re, m.im, t.dc_e[i]); }
-    __device__ static xl DZ(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xl_of(m.re, m.im, t.dz_e[i]); }
-    // orbit point i: the register's Xrange value where the binary64 orbit
-    // underflows, the binary64 value elsewhere
-    __device__ static xl ZR(const Tables &t, int32_t i) {
-        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
-        if (j >= 0) return xl_of(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]);
-        return xl_of(t.Zn[i]);
-    }
-
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        s.c = c_from_pix_xl(a, a.pix[2 * i], a.pix[2 * i + 1]);
-        s.z = xl_zero(); s.d = xl_zero(); s.h = xl_zero();
-        s.zs = zc();
-        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
-    }
-
-    __device__ static int step(State &s, const KArgs &a, Counters &k) {
-        const Tables &t = a.t;
-        while (t.bla_on && (s.w & 7) == 0) {
-            int32_t idx;
-            int32_t st = bla_lookup(t, s.zs.re, s.zs.im, s.w, idx);
-            k.tries++;
-            if (st == 0) break;
-            {
-                const double *Mb = t.M + 4 * idx;
-                cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
-                s.n += st;
-                s.w += st;
-                s.z = s.z * A + s.c * B;
-                s.zs = to_std(s.z);
-                if (a.calc_dc) s.d = s.d * A;
-                if (a.calc_dz) s.h = s.h * A;
-                k.bla++; k.bla_it += st;
-            }
-        }
-        s.n += 1; k.plain++;
-        const xl Zr = ZR(t, s.w);
-        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xl_zero() : DC(t, s.w));
-        if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ(t, 0) : DZ(t, s.w));
-        s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
-        s.zs = to_std(s.z);
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        if (s.n >= a.max_iter) return 0;
-        if (a.calc_dz) {
-            xl nx = s.nul ? DZ(t, 0) : (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
-            if (lt(abs2(s.h + nx), a.eps2)) return 2;
-        }
-        const cplx ZZ = s.zs + t.Zn[s.w];
-        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
-        if (s.w >= t.div_iter - 1) {
-            s.zs = ZZ;
-            s.z = xl_of(ZZ);
-            if (a.calc_dc) s.d = s.d + DC(t, s.w);
-            if (a.calc_dz) {
-                if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
-                s.nul = true;
-            }
-            s.w = 0;
-            k.reb++;
-            return -1;
-        }
-        s.dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
-        if (s.dyn) {
-            const xl ZZx = s.z + ZR(t, s.w);
-            if (le(abs2(ZZx), abs2(s.z))) {
-                s.z = ZZx;
-                s.zs = to_std(ZZx);
-                if (a.calc_dc) s.d = s.d + DC(t, s.w);
-                if (a.calc_dz) {
-                    if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
-                    s.nul = true;
-                }
-                s.w = 0;
-                k.reb++;
-            }
-        }
-        return -1;
-    }
-
-    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.ld;
-        cplx z = to_std(s.z) + a.t.Zn[s.w];
-        a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
-        int f = 1;
-        // perturbation.py:1377-1380 writes zn and dzndc back only; the
-        // standard dzndz field keeps the 0 written by numba_initialize
-        if (a.calc_dz) { a.Z[2 * (n + i)] = 0.; a.Z[2 * (n + i) + 1] = 0.; f = 2; }
-        if (a.calc_dc) {
-            cplx d = to_std(s.d + DC(a.t, s.w));
-            a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
-        }
-        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
-    }
-};
-
-// --------------------------------------------------------------------------
-// Burning-ship family, binary64 deltas
-template <int FL>
-struct BsStd {
-    struct State { double x, y, P[4], a, b; int32_t w, n; bool dyn; };
-
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        xc cx = renorm_c(a.dx_m * (a.lin[0] * a.pix[2 * i] + a.lin[1] * a.pix[2 * i + 1]),
-                         a.dx_m * (a.lin[2] * a.pix[2 * i] + a.lin[3] * a.pix[2 * i + 1]), a.dx_e);
-        s.a = to_std(mkf(cx.re, cx.e) + mkf(a.drift_m[0], a.drift_e[0]));
-        s.b = to_std(mkf(cx.im, cx.e) + mkf(a.drift_m[1], a.drift_e[1]));
-        s.x = 0.; s.y = 0.; s.P[0] = s.P[1] = s.P[2] = s.P[3] = 0.;
-        s.w = 0; s.n = 0; s.dyn = true;
-    }
-
-    __device__ static int step(State &s, const KArgs &a, Counters &k) {
-        const Tables &t = a.t;
-        while (t.bla_on && (s.w & 7) == 0) {
-            int32_t idx;
-            int32_t st = bla_lookup(t, s.x, s.y, s.w, idx);
-            k.tries++;
-            if (st == 0) break;
-            {
-                const double *M = t.M + 8 * idx;
-                s.n += st; s.w += st;
-                double nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
-                double ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
-                s.x = nx; s.y = ny;
-                if (a.calc_dc) {
-                    double p0 = M[0] * s.P[0] + M[1] * s.P[2];
-                    double p1 = M[0] * s.P[1] + M[1] * s.P[3];
-                    double p2 = M[2] * s.P[0] + M[3] * s.P[2];
-                    double p3 = M[2] * s.P[1] + M[3] * s.P[3];
-                    s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
-                }
-                k.bla++; k.bla_it += st;
-            }
-        }
-        s.n += 1; k.plain++;
-        const cplx R = t.Zn[s.w];
-        if (a.calc_dc) {
-            double Rd[4] = {0., 0., 0., 0.};
-            if (!s.dyn) { Rd[0] = t.bs[0][s.w]; Rd[1] = t.bs[1][s.w]; Rd[2] = t.bs[2][s.w]; Rd[3] = t.bs[3][s.w]; }
-            bs_hess<FL>(s.x, s.y, s.P, R.re, R.im, Rd);
-        }
-        bs_zn<FL>(s.x, s.y, R.re, R.im, s.a, s.b);
-        if (s.n >= a.max_iter) return 0;
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        const cplx R1 = t.Zn[s.w];
-        double XX = s.x + R1.re, YY = s.y + R1.im;
-        if (XX * XX + YY * YY > a.M2) return 1;
-        bool reb = (s.w >= t.div_iter - 1);
-        if (!reb) {
-            s.dyn = (fabs(XX) <= fabs(s.x)) && (fabs(YY) <= fabs(s.y));
-            reb = s.dyn;
-        }
-        if (reb) {
-            s.x = XX; s.y = YY;
-            if (a.calc_dc)
-                for (int q = 0; q < 4; q++) s.P[q] += t.bs[q][s.w];
-            s.w = 0;
-            k.reb++;
-        }
-        return -1;
-    }
-
-    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.ld;
-        const cplx R = a.t.Zn[s.w];
-        a.Z[i] = s.x + R.re; a.Z[n + i] = s.y + R.im;
-        if (a.calc_dc)
-            for (int q = 0; q < 4; q++) a.Z[(2 + q) * n + i] = s.P[q] + a.t.bs[q][s.w];
-        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
-    }
-};
-
-// Burning-ship family, Xrange deltas (xl.hpp reals)
-template <int FL>
-struct BsXr {
-    struct State { xlf x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; };
-
-    __device__ static xlf PATH(const Tables &t, int q, int32_t i) { return xlf_of(t.bs[q][i], t.bs_e[q][i]); }
-    __device__ static void ZR(const Tables &t, int32_t i, xlf &X, xlf &Y) {
-        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
-        if (j >= 0) { X = xlf_of(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xlf_of(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
-        else { const cplx R = t.Zn[i]; X = xlf_of(R.re, 0); Y = xlf_of(R.im, 0); }
-    }
-
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        const double px = a.pix[2 * i], py = a.pix[2 * i + 1];
-        s.a = xlf_of(a.dx_m * (a.lin[0] * px + a.lin[1] * py), a.dx_e) + xlf_of(a.drift_m[0], a.drift_e[0]);
-        s.b = xlf_of(a.dx_m * (a.lin[2] * px + a.lin[3] * py), a.dx_e) + xlf_of(a.drift_m[1], a.drift_e[1]);
-        const xlf z0 = xlf_zero();
-        s.x = z0; s.y = z0; s.P[0] = s.P[1] = s.P[2] = s.P[3] = z0;
-        s.xs = 0.; s.ys = 0.;
-        s.w = 0; s.n = 0; s.dyn = true;
-    }
-
-    __device__ static int step(State &s, const KArgs &a, Counters &k) {
-        const Tables &t = a.t;
-        while (t.bla_on && (s.w & 7) == 0) {
-            int32_t idx;
-            int32_t st = bla_lookup(t, s.xs, s.ys, s.w, idx);
-            k.tries++;
-            if (st == 0) break;
-            {
-                const double *M = t.M + 8 * idx;
-                s.n += st; s.w += st;
-                xlf nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
-                xlf ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
-                s.x = nx; s.y = ny;
-                s.xs = to_std(s.x); s.ys = to_std(s.y);
-                if (a.calc_dc) {
-                    xlf p0 = M[0] * s.P[0] + M[1] * s.P[2];
-                    xlf p1 = M[0] * s.P[1] + M[1] * s.P[3];
-                    xlf p2 = M[2] * s.P[0] + M[3] * s.P[2];
-                    xlf p3 = M[2] * s.P[1] + M[3] * s.P[3];
-                    s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
-                }
-                k.bla++; k.bla_it += st;
-            }
-        }
-        s.n += 1; k.plain++;
-        xlf X, Y;
-        ZR(t, s.w, X, Y);
-        if (a.calc_dc) {
-            const xlf z0 = xlf_zero();
-            xlf Rd[4] = {z0, z0, z0, z0};
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = PATH(t, q, s.w);
-            bs_hess<FL>(s.x, s.y, s.P, X, Y, Rd);
-        }
-        bs_zn<FL>(s.x, s.y, X, Y, s.a, s.b);
-        s.xs = to_std(s.x); s.ys = to_std(s.y);
-        if (s.n >= a.max_iter) return 0;
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        const cplx R1 = t.Zn[s.w];
-        double XX = s.xs + R1.re, YY = s.ys + R1.im;
-        if (XX * XX + YY * YY > a.M2) return 1;
-        if (s.w >= t.div_iter - 1) {
-            s.xs = XX; s.ys = YY;
-            s.x = xlf_of(XX, 0); s.y = xlf_of(YY, 0);
-            if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
-            s.w = 0;
-            k.reb++;
-            return -1;
-        }
-        s.dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
-        if (s.dyn) {
-            xlf X1, Y1;
-            ZR(t, s.w, X1, Y1);
-            const xlf XXx = s.x + X1, YYx = s.y + Y1;
-            if (le(XXx * XXx + YYx * YYx, s.x * s.x + s.y * s.y)) {
-                s.x = XXx; s.y = YYx;
-                s.xs = to_std(XXx); s.ys = to_std(YYx);
-                if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
-                s.w = 0;
-                k.reb++;
-            }
-        }
-        return -1;
-    }
-
-    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.ld;
-        const cplx R = a.t.Zn[s.w];
-        a.Z[i] = to_std(s.x + R.re); a.Z[n + i] = to_std(s.y + R.im);
-        if (a.calc_dc)
-            for (int q = 0; q < 4; q++) a.Z[(2 + q) * n + i] = to_std(s.P[q] + PATH(a.t, q, s.w));
-        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
-    }
-};
 
 // --------------------------------------------------------------------------
 // BLA build: level 0 + three compressed levels per block of 8 iterations,
@@ -527,11 +146,10 @@ __global__ void k_bla_merge(int32_t comp, int32_t stg, double kc, double eps, do
 
 // --------------------------------------------------------------------------
 // dispatch
-typedef void (*tile_fn)(const KArgs);
-
 template <class Model> static tile_fn fn_of() { return k_tile<Model>; }
 
 static tile_fn select_tile(const LaunchCfg &c) {
+    if (c.orbit) return select_tile_orbit(c);
     if (c.kind == KIND_M2) return c.xr ? fn_of<HoloXr<KIND_M2, 2>>() : fn_of<HoloStd<KIND_M2, 2>>();
     if (c.kind == KIND_MN) {
         switch (c.degree) {

@@ -135,6 +135,42 @@ FSG_DEV cplx to_std(xl a) {
 }
 FSG_DEV double to_std(xlf a) { return ldexp(a.m, max(a.e, -2200)); }
 
+// ---- lazily normalised intermediates ---------------------------------------
+// Within one iteration, values are combined without renormalising: their
+// mantissas start in [1, 2) (state, orbit point) or within 2^+-100 (the
+// reference's derivative paths) and a few products and sums cannot take
+// them near the binary64 limits, so every operation still rounds exactly
+// as the reference's.  Only a zero result needs care: it takes XL_ZERO_E so
+// that it never sets the exponent of a later sum.  The state is
+// renormalised (xl_of) once per iteration.
+struct xz { double re, im; int32_t e; };
+
+FSG_DEV xz xz_mk(double re, double im, int32_t e) {
+    xz r;
+    r.re = re; r.im = im;
+    r.e = (fmax(fabs(re), fabs(im)) == 0.) ? XL_ZERO_E : e;
+    return r;
+}
+FSG_DEV xz as_z(xl a) { xz r; r.re = a.re; r.im = a.im; r.e = a.e; return r; }
+FSG_DEV xz xz_zero() { xz r; r.re = 0.; r.im = 0.; r.e = XL_ZERO_E; return r; }
+FSG_DEV xl xl_of(xz a) { return xl_n(a.re, a.im, a.e); }
+
+FSG_DEV xz operator+(xz a, xz b) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
+    return xz_mk(ldexp(a.re, sa) + ldexp(b.re, sb), ldexp(a.im, sa) + ldexp(b.im, sb), E);
+}
+FSG_DEV xz operator*(xz a, xz b) {
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    return xz_mk(ac - bd, ad + bc, a.e + b.e);
+}
+FSG_DEV xz operator*(xz a, cplx b) {
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    return xz_mk(ac - bd, ad + bc, a.e);
+}
+FSG_DEV xz muls(double s, xz a) { return xz_mk(s * a.re, s * a.im, a.e); }
+FSG_DEV xlf abs2(xz a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
+
 // burning-ship helpers on Xrange reals (burning_ship.py:19-60)
 FSG_DEV xlf diffabs(xlf X, xlf x) {
     if (ge(X, 0.)) { if (ge(X + x, 0.)) return x; return -(2. * X + x); }

@@ -33,13 +33,15 @@ class _HolomorphicBase(PerturbationFractal):
 
     def _holo_calc(self, *, calc_name, subset, max_iter, M_divergence, epsilon_stationnary,
                    BLA_eps, interior_detect, calc_dzndc, calc_orbit, backshift):
-        if calc_orbit:
-            raise NotImplementedError("calc_orbit is not supported by the GPU engine")
+        if calc_orbit and backshift < 1:
+            raise ValueError("calc_orbit needs backshift > 0")
         complex_codes = ["zn"]
         if interior_detect:
             complex_codes.append("dzndz")
         if calc_dzndc:
             complex_codes.append("dzndc")
+        if calc_orbit:
+            complex_codes.append("zn_orbit")      # mandelbrot_M2.py:501-506
         int_codes = ["ref_cycle_iter"]
         stop_codes = ["max_iter", "divergence", "stationnary"]
         BLA_activated = (BLA_eps is not None) and bool(self.dx < settings.newton_zoom_level)
@@ -51,6 +53,7 @@ class _HolomorphicBase(PerturbationFractal):
                 instance.codes = (complex_codes, int_codes, stop_codes)
                 instance.calc_dZndz = interior_detect
                 instance.calc_dZndc = calc_dzndc
+                instance.backshift = backshift if calc_orbit else None
             return impl
 
         def initialize():
@@ -63,6 +66,7 @@ class _HolomorphicBase(PerturbationFractal):
                 "epsilon_stationnary": epsilon_stationnary,
                 "BLA_activated": BLA_activated, "BLA_eps": BLA_eps,
                 "calc_dzndc": calc_dzndc, "calc_dzndz": interior_detect,
+                "calc_orbit": calc_orbit, "backshift": backshift,
                 "xr_detect_activated": self.xr_detect_activated,
             }
 
@@ -202,11 +206,13 @@ class Perturbation_burning_ship(PerturbationFractal):
                      calc_hessian: bool = True, calc_orbit: bool = False, backshift: int = 0):
         """Perturbation iterations for the burning-ship family
         (burning_ship.py:971-1130)."""
-        if calc_orbit:
-            raise NotImplementedError("calc_orbit is not supported by the GPU engine")
+        if calc_orbit and backshift < 1:
+            raise ValueError("calc_orbit needs backshift > 0")
         complex_codes = ["xn", "yn"]
         if calc_hessian:
             complex_codes += ["dxnda", "dxndb", "dynda", "dyndb"]
+        if calc_orbit:
+            complex_codes += ["xn_orbit", "yn_orbit"]   # burning_ship.py:1028-1034
         int_codes = ["ref_cycle_iter"]
         stop_codes = ["max_iter", "divergence"]
         BLA_activated = (BLA_eps is not None) and bool(self.dx < settings.newton_zoom_level)
@@ -216,7 +222,7 @@ class Perturbation_burning_ship(PerturbationFractal):
                 instance.complex_type = np.float64
                 instance.potential_M = M_divergence
                 instance.codes = (complex_codes, int_codes, stop_codes)
-                instance.backshift = None
+                instance.backshift = backshift if calc_orbit else None
             return impl
 
         def initialize():
@@ -227,6 +233,7 @@ class Perturbation_burning_ship(PerturbationFractal):
                 "max_iter": max_iter, "M_divergence": M_divergence,
                 "BLA_activated": BLA_activated, "BLA_eps": BLA_eps,
                 "calc_dzndc": calc_hessian, "calc_dzndz": False,
+                "calc_orbit": calc_orbit, "backshift": backshift,
                 "xr_detect_activated": self.xr_detect_activated,
             }
 

@@ -303,6 +303,8 @@ class PerturbationFractal(Fractal):
             "calc_dzndc": int(spec["calc_dzndc"]),
             "interior_detect": int(spec.get("calc_dzndz", False)),
             "precision": int(spec.get("precision", PRECISIONS[settings.gpu_precision])),
+            "calc_orbit": int(bool(spec.get("calc_orbit", False))),
+            "backshift": int(spec.get("backshift", 0) or 0) if spec.get("calc_orbit") else 0,
         }
         arrays = {
             "Zn_path": pd["Zn_path"].view(np.float64),

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 1
+#define FSG_ABI_VERSION 2
 
 /* model kinds */
 #define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
@@ -84,6 +84,10 @@ typedef struct fsg_ref_desc {
     int32_t calc_dzndc;      /* holomorphic calc_dzndc; burning ship calc_hessian */
     int32_t interior_detect; /* holomorphic calc_dzndz (interior detection) */
     int32_t precision;       /* FSG_PREC_* */
+    int32_t calc_orbit;      /* keep the orbit point at exit - backshift (perturbation.py:1096-1101,
+                                1266-1273, 1390-1396): one more complex field zn_orbit (holomorphic)
+                                or two float64 fields xn_orbit, yn_orbit (burning ship); FP64 only */
+    int32_t backshift;       /* > 0 when calc_orbit */
     int32_t pad2;
 } fsg_ref_desc;
 
@@ -135,8 +139,8 @@ int fsg_ctx_destroy(fsg_ctx *ctx);
  * numba_cycle_call (perturbation.py:414-428), cycle_dep_args =
  * (c_pix, Z, U, stop_reason, stop_iter) of core.py:2044-2075:
  *   c_pix       [2 n] complex128 pixel offsets in units of dx
- *   Z           holomorphic: [n_Z, n] complex128 (zn, [dzndz], [dzndc]);
- *               burning ship: [n_Z, n] float64 (xn, yn, [4 Jacobian])
+ *   Z           holomorphic: [n_Z, n] complex128 (zn, [dzndz], [dzndc], [zn_orbit]);
+ *               burning ship: [n_Z, n] float64 (xn, yn, [4 Jacobian], [xn_orbit, yn_orbit])
  *   U           [n] int32 (ref_cycle_iter)
  *   stop_reason [n] int8
  *   stop_iter   [n] int32

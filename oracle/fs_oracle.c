@@ -284,6 +284,7 @@ typedef struct {
     double M2;                   /* M_divergence ** 2 */
     double eps2;                 /* epsilon_stationnary ** 2 */
     int32_t calc_dzndc, calc_dzndz;
+    int32_t calc_orbit; int64_t backshift;   /* orbit point at exit - backshift */
     /* derived (filled by or_ctx_prepare) */
     double *dzndc_m; int32_t *dzndc_e;     /* holo: [2L] (+ [L] exps) */
     double *dzndz_m; int32_t *dzndz_e;     /* holo: [2(L+1)] (+ [L+1]) */
@@ -714,8 +715,43 @@ static inline xrc c_from_pix_holo(const or_ctx *k, double px, double py) {
 
 /* numba_iterate(...).numba_impl, perturbation.py:1065-1400 (holomorphic).
  * Z fields: zn, [dzndz], [dzndc] (mandelbrot_M2.py:482-506). */
+/* calc_orbit: perturbation.py:1096-1101 (init), 1266-1273 (record at the
+ * escape test, after plain steps only), 1390-1396 (finish: iterate the
+ * full recurrence in binary64 from the last-but-one record to
+ * n - backshift, with c + Zn_path[1]). */
+typedef struct { int64_t div_shift, i1, i2; cpx z1, z2; } orbit_rec;
+
+static inline void orbit_record(orbit_rec *o, int64_t n, int64_t backshift, cpx ZZ) {
+    int64_t div = n / backshift;
+    if (div > o->div_shift) {
+        o->div_shift = div;
+        o->i2 = o->i1; o->z2 = o->z1;
+        o->i1 = n; o->z1 = ZZ;
+    }
+}
+
+/* zn ** d + c for the orbit finish: M2_iterate (mandelbrot_M2.py:14-15);
+ * Mn_iterate (mandelbrot_Mn.py:13-17) as binary powering */
+static cpx zn_iterate(const or_ctx *k, cpx z, cpx c) {
+    if (k->kind == OR_KIND_M2) return c_add(c_mul(z, z), c);
+    cpx r = C(1., 0.), p = z;
+    for (int64_t m = k->degree; m > 0; m >>= 1) {
+        if (m & 1) r = c_mul(r, p);
+        if (m > 1) p = c_mul(p, p);
+    }
+    return c_add(r, c);
+}
+
+static cpx orbit_finish(const or_ctx *k, orbit_rec *o, int64_t n, cpx c) {
+    cpx zo = o->z2;
+    const cpx CC = c_add(c, Zref(k, 1));
+    while (o->i2 < n - k->backshift) { zo = zn_iterate(k, zo, CC); o->i2++; }
+    return zo;
+}
+
 static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dzndz,
-                            cpx *out_dzndc, int32_t *U, int8_t *stop) {
+                            cpx *out_dzndc, cpx *out_orbit, int32_t *U, int8_t *stop) {
+    orbit_rec orb = {0, 0, 0, {0., 0.}, {0., 0.}};
     const int64_t L = k->L, order = k->order, div_iter = k->div_iter;
     const int dc = k->calc_dzndc, dz = k->calc_dzndz;
     int64_t w = 0, n = 0;
@@ -762,6 +798,7 @@ static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dznd
             }
             cpx ZZ = c_add(z, Zref(k, w));
             double fsq = ZZ.re * ZZ.re + ZZ.im * ZZ.im;
+            if (k->calc_orbit) orbit_record(&orb, n, k->backshift, ZZ);
             if (fsq > k->M2) { *stop = 1; break; }
             if (w >= div_iter - 1) {
                 z = ZZ;
@@ -789,6 +826,7 @@ static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dznd
         *out_z = c_add(z, Zref(k, w));
         if (dc) *out_dzndc = c_add(d, D[w]);
         if (dz) *out_dzndz = h;
+        if (k->calc_orbit) *out_orbit = orbit_finish(k, &orb, n, c);
         return n;
     }
 
@@ -841,6 +879,7 @@ static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dznd
         cpx Zn1 = Zref(k, w);
         cpx ZZ = c_add(zs, Zn1);
         double fsq = ZZ.re * ZZ.re + ZZ.im * ZZ.im;
+        if (k->calc_orbit) orbit_record(&orb, n, k->backshift, ZZ);
         if (fsq > k->M2) { *stop = 1; break; }
         if (w >= div_iter - 1) {
             zs = ZZ;
@@ -875,6 +914,7 @@ static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dznd
     /* perturbation.py:1377-1380 writes back zn and dzndc only: in Xrange mode
      * the standard dzndz field keeps its initial 0 (numba_initialize). */
     if (dz) *out_dzndz = C(0., 0.);
+    if (k->calc_orbit) *out_orbit = orbit_finish(k, &orb, n, xrc_std(c_xr));
     return n;
 #undef DXR
 #undef DZXR
@@ -1105,8 +1145,32 @@ static inline void bs_hess_xr(int fl, bs_xr *Z, xrf X, xrf Y, xrf Xa, xrf Xb, xr
 }
 
 /* numba_iterate_BS(...).numba_impl, perturbation.py:1468-1791 */
+/* BS_iterate (burning_ship.py:82-123): the full recurrence per flavour */
+static void bs_iterate_full(int fl, double *x, double *y, double A, double B) {
+    const double X = *x, Y = *y;
+    double nx, ny;
+    switch (fl) {
+        case 1: nx = X * X - Y * Y + A; ny = 2. * fabs(X * Y) - B; break;
+        case 2: nx = X * X - Y * Y + A; ny = 2. * X * fabs(Y) - B; break;
+        case 3: nx = X * X - Y * fabs(Y) + A; ny = 2. * X * Y - B; break;
+        case 4: nx = fabs(X * X - Y * Y) + A; ny = 2. * X * Y - B; break;
+        default: nx = fabs(X * X - Y * Y) + A; ny = 2. * fabs(X * Y) - B; break;
+    }
+    *x = nx; *y = ny;
+}
+
+/* orbit finish of the BS loop, perturbation.py:1778-1787 */
+static void bs_orbit_finish(const or_ctx *k, orbit_rec *o, int64_t n, double a, double b, double *out) {
+    double x = o->z2.re, y = o->z2.im;
+    const cpx R1 = Zref(k, 1);
+    const double AA = a + R1.re, BB = b + R1.im;
+    while (o->i2 < n - k->backshift) { bs_iterate_full(k->flavor, &x, &y, AA, BB); o->i2++; }
+    out[0] = x; out[1] = y;
+}
+
 static int64_t iterate_bs(const or_ctx *k, double px, double py, double *out6,
                           int32_t *U, int8_t *stop) {
+    orbit_rec orb = {0, 0, 0, {0., 0.}, {0., 0.}};
     const int64_t L = k->L, order = k->order, div_iter = k->div_iter;
     const int hs = k->calc_dzndc;
     const int fl = k->flavor;
@@ -1163,6 +1227,7 @@ static int64_t iterate_bs(const or_ctx *k, double px, double py, double *out6,
             cpx R1 = Zref(k, w);
             double XX = Z.x + R1.re, YY = Z.y + R1.im;
             double fsq = XX * XX + YY * YY;
+            if (k->calc_orbit) orbit_record(&orb, n, k->backshift, C(XX, YY));
             if (fsq > k->M2) { *stop = 1; break; }
             if (w >= div_iter - 1) {
                 Z.x = XX; Z.y = YY;
@@ -1185,6 +1250,7 @@ static int64_t iterate_bs(const or_ctx *k, double px, double py, double *out6,
             out6[2] = Z.dxa + PA[w]; out6[3] = Z.dxb + PB[w];
             out6[4] = Z.dya + PC[w]; out6[5] = Z.dyb + PD[w];
         }
+        if (k->calc_orbit) bs_orbit_finish(k, &orb, n, a, b, out6 + 6);
         return n;
     }
 
@@ -1243,6 +1309,7 @@ static int64_t iterate_bs(const or_ctx *k, double px, double py, double *out6,
         cpx R1 = Zref(k, w);
         double XX = xs + R1.re, YY = ys + R1.im;
         double fsq = XX * XX + YY * YY;
+        if (k->calc_orbit) orbit_record(&orb, n, k->backshift, C(XX, YY));
         if (fsq > k->M2) { *stop = 1; break; }
         if (w >= div_iter - 1) {
             xs = XX; ys = YY;
@@ -1281,6 +1348,7 @@ static int64_t iterate_bs(const or_ctx *k, double px, double py, double *out6,
         out6[2] = xrf_std(XA(Z.dxa, PX(0, w))); out6[3] = xrf_std(XA(Z.dxb, PX(1, w)));
         out6[4] = xrf_std(XA(Z.dya, PX(2, w))); out6[5] = xrf_std(XA(Z.dyb, PX(3, w)));
     }
+    if (k->calc_orbit) bs_orbit_finish(k, &orb, n, a, b, out6 + 6);
     return n;
 #undef PX
 }
@@ -1302,6 +1370,7 @@ typedef struct {
     int64_t max_iter;
     double M2, eps2;
     int32_t calc_dzndc, calc_dzndz;
+    int32_t calc_orbit, backshift;
 } or_desc;
 
 void *or_ctx_create(const or_desc *d) {
@@ -1317,6 +1386,8 @@ void *or_ctx_create(const or_desc *d) {
     k->bla_on = d->bla_on; k->bla_eps = d->bla_eps;
     k->max_iter = d->max_iter; k->M2 = d->M2; k->eps2 = d->eps2;
     k->calc_dzndc = d->calc_dzndc; k->calc_dzndz = (k->kind == OR_KIND_BS) ? 0 : d->calc_dzndz;
+    k->calc_orbit = d->calc_orbit; k->backshift = d->backshift;
+    if (k->calc_orbit && k->backshift <= 0) { free(k); return NULL; }
     int rc = (k->kind == OR_KIND_BS) ? bs_paths(k) : holo_paths(k);
     if (rc != 0) { free(k); return NULL; }
     if (k->bla_on) {
@@ -1365,13 +1436,14 @@ int or_cycles(void *p, const double *c_pix, int64_t npts, double *Z, int32_t *U,
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
     if (k->kind == OR_KIND_BS) {
-        int nz = 2 + (k->calc_dzndc ? 4 : 0);
+        const int nj = 2 + (k->calc_dzndc ? 4 : 0);
         #pragma omp parallel for schedule(dynamic, 16)
         for (int64_t i = 0; i < npts; i++) {
-            double out6[6] = {0, 0, 0, 0, 0, 0};
+            double out8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             int8_t st = -1;
-            int64_t n = iterate_bs(k, c_pix[2 * i], c_pix[2 * i + 1], out6, &U[i], &st);
-            for (int f = 0; f < nz; f++) Z[f * npts + i] = out6[f];
+            int64_t n = iterate_bs(k, c_pix[2 * i], c_pix[2 * i + 1], out8, &U[i], &st);
+            for (int f = 0; f < nj; f++) Z[f * npts + i] = out8[f];
+            if (k->calc_orbit) { Z[nj * npts + i] = out8[6]; Z[(nj + 1) * npts + i] = out8[7]; }
             stop_iter[i] = (int32_t)n;
             stop_reason[i] = st;
         }
@@ -1379,15 +1451,17 @@ int or_cycles(void *p, const double *c_pix, int64_t npts, double *Z, int32_t *U,
     }
     int f_dz = k->calc_dzndz ? 1 : -1;
     int f_dc = k->calc_dzndc ? (k->calc_dzndz ? 2 : 1) : -1;
+    int f_orb = k->calc_orbit ? 1 + (k->calc_dzndz ? 1 : 0) + (k->calc_dzndc ? 1 : 0) : -1;
     #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < npts; i++) {
         xrc cx = c_from_pix_holo(k, c_pix[2 * i], c_pix[2 * i + 1]);
-        cpx oz = C(0, 0), oh = C(0, 0), od = C(0, 0);
+        cpx oz = C(0, 0), oh = C(0, 0), od = C(0, 0), oo = C(0, 0);
         int8_t st = -1;
-        int64_t n = iterate_holo(k, cx, &oz, &oh, &od, &U[i], &st);
+        int64_t n = iterate_holo(k, cx, &oz, &oh, &od, &oo, &U[i], &st);
         Z[2 * i] = oz.re; Z[2 * i + 1] = oz.im;
         if (f_dz > 0) { Z[2 * (f_dz * npts + i)] = oh.re; Z[2 * (f_dz * npts + i) + 1] = oh.im; }
         if (f_dc > 0) { Z[2 * (f_dc * npts + i)] = od.re; Z[2 * (f_dc * npts + i) + 1] = od.im; }
+        if (f_orb > 0) { Z[2 * (f_orb * npts + i)] = oo.re; Z[2 * (f_orb * npts + i) + 1] = oo.im; }
         stop_iter[i] = (int32_t)n;
         stop_reason[i] = st;
     }

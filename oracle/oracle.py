@@ -35,6 +35,7 @@ class _Desc(ctypes.Structure):
         ("max_iter", ctypes.c_int64),
         ("M2", ctypes.c_double), ("eps2", ctypes.c_double),
         ("calc_dzndc", ctypes.c_int32), ("calc_dzndz", ctypes.c_int32),
+        ("calc_orbit", ctypes.c_int32), ("backshift", ctypes.c_int32),
     ]
 
 
@@ -96,6 +97,8 @@ class OracleCtx:
         d.M2 = fields["M_divergence"] * fields["M_divergence"]
         d.eps2 = fields["epsilon_stationnary"] * fields["epsilon_stationnary"]
         d.calc_dzndc, d.calc_dzndz = fields["calc_dzndc"], fields["interior_detect"]
+        d.calc_orbit = int(fields.get("calc_orbit", 0))
+        d.backshift = int(fields.get("backshift", 0))
         self.fields = fields
         self.kind = fields["kind"]
         self._h = lib().or_ctx_create(ctypes.byref(d))
@@ -111,9 +114,10 @@ class OracleCtx:
 
     def n_z(self):
         f = self.fields
+        orb = bool(f.get("calc_orbit", 0))
         if self.kind == 2:
-            return 2 + (4 if f["calc_dzndc"] else 0)
-        return 1 + int(bool(f["calc_dzndc"])) + int(bool(f["interior_detect"]))
+            return 2 + (4 if f["calc_dzndc"] else 0) + (2 if orb else 0)
+        return 1 + int(bool(f["calc_dzndc"])) + int(bool(f["interior_detect"])) + int(orb)
 
     def cycles(self, c_pix, nthreads=0):
         c_pix = np.ascontiguousarray(c_pix, dtype=np.complex128)

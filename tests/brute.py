@@ -4,8 +4,14 @@ import mpmath
 
 
 def pixel_c(f, pix):
+    """centre + dx * (lin_mat . pix): rotation and skew included
+    (lin_proj_impl, perturbation.py:2665-2672)."""
+    m = f.lin_mat
+    px, py = float(pix.real), float(pix.imag)
     with mpmath.workprec(f.prec + 64):
-        return mpmath.mpc(f.x + mpmath.mpf(pix.real) * f.dx, f.y + mpmath.mpf(pix.imag) * f.dx)
+        ox = mpmath.mpf(float(m[0, 0])) * px + mpmath.mpf(float(m[0, 1])) * py
+        oy = mpmath.mpf(float(m[1, 0])) * px + mpmath.mpf(float(m[1, 1])) * py
+        return mpmath.mpc(f.x + ox * f.dx, f.y + oy * f.dx)
 
 
 def holo_escape(f, pix, degree, max_iter, M, prec=600):
@@ -35,3 +41,24 @@ def bs_escape(f, pix, flavor, max_iter, M, prec=600):
             if x * x + y * y > M2:
                 break
         return n, complex(x, y)
+
+
+def holo_point(f, pix, degree, k, prec=600):
+    """z_k of the pixel's own orbit (z_0 = 0)."""
+    c = pixel_c(f, pix)
+    with mpmath.workprec(prec):
+        z = mpmath.mpc(0)
+        for _ in range(max(k, 0)):
+            z = z ** degree + c if degree > 2 else z * z + c
+        return complex(z)
+
+
+def bs_point(f, pix, flavor, k, prec=600):
+    from fractalshades_amd.mpmath_loop import bs_iterate
+    c = pixel_c(f, pix)
+    with mpmath.workprec(prec):
+        a, b = c.real, c.imag
+        x = y = mpmath.mpf(0)
+        for _ in range(max(k, 0)):
+            x, y = bs_iterate(flavor, x, y, a, b)
+        return complex(x, y)

@@ -2,16 +2,14 @@
 buffer layout."""
 import numpy as np
 
+from fractalshades_amd._native import n_z_fields
+
 
 def run_gpu(args, c_pix):
     kind = args.fields["kind"]
     n = c_pix.shape[0]
-    if kind == 2:
-        nz = 2 + (4 if args.fields["calc_dzndc"] else 0)
-        Z = np.zeros((nz, n), dtype=np.float64)
-    else:
-        nz = 1 + int(bool(args.fields["calc_dzndc"])) + int(bool(args.fields["interior_detect"]))
-        Z = np.zeros((nz, n), dtype=np.complex128)
+    nz = n_z_fields(args.fields)
+    Z = np.zeros((nz, n), dtype=np.float64 if kind == 2 else np.complex128)
     U = np.zeros((1, n), dtype=np.int32)
     sr = -np.ones((1, n), dtype=np.int8)
     si = np.zeros((1, n), dtype=np.int32)

@@ -100,3 +100,45 @@ def test_bs_bitexact(name, flavor, xr):
     ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
     got, stats = run_gpu(args, c_pix)
     assert_same(got, ref)
+
+
+@pytest.mark.parametrize("case", [
+    ("M2_E20", 0, 3, 1), ("M2_E20", 1, 3, 1), ("M2_divref5", 0, 1, 1), ("M2_E11", 1, 7, 0),
+])
+def test_calc_orbit_bitexact(case):
+    """calc_orbit / backshift: the extra zn_orbit field, BLA on and off,
+    binary64 and Xrange deltas."""
+    name, xr, backshift, bla = case
+    f = frames.make(name, nx=48, calc_orbit=True, backshift=backshift)
+    args, c_pix = frames.frame_inputs(f)
+    assert args.fields["calc_orbit"] == 1 and args.fields["backshift"] == backshift
+    args = with_fields(args, xr_mode=xr, bla_on=bla)
+    print(f"case orbit {case}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, _ = run_gpu(args, c_pix)
+    assert got[0].shape[0] == f._calc_data["t"]["state"].codes[0].__len__()
+    assert_same(got, ref)
+
+
+@pytest.mark.parametrize("d,xr", [(3, 0), (4, 1)])
+def test_calc_orbit_mn_bitexact(d, xr):
+    f = frames.mn_frame(d, nx=40, calc_orbit=True, backshift=2)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case orbit Mn d={d} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, _ = run_gpu(args, c_pix)
+    assert_same(got, ref)
+
+
+@pytest.mark.parametrize("flavor,xr", [(None, 0), ("Celtic", 1)])
+def test_calc_orbit_bs_bitexact(flavor, xr):
+    f = frames.make("BS_ex14", nx=40, model_kw={"flavor": flavor} if flavor else None,
+                    max_iter=6000, calc_orbit=True, backshift=4)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case orbit BS {flavor} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, _ = run_gpu(args, c_pix)
+    assert got[0].shape[0] == 8
+    assert_same(got, ref)

@@ -75,3 +75,86 @@ def test_interior_detection_flags_bounded_pixels():
     for i in np.flatnonzero(sr[0] == 1)[:4]:
         n, z, dz = brute.holo_escape(f, c_pix[i], 2, args.fields["max_iter"], 1e3, prec=300)
         assert abs(int(n) - int(si[0, i])) <= 1
+
+
+@pytest.mark.parametrize("backshift", [1, 3, 7])
+def test_calc_orbit_vs_brute_force(backshift):
+    """calc_orbit (perturbation.py:1096-1101, 1266-1273, 1390-1396): the
+    extra field is the pixel's orbit point at exit - backshift.  BLA off,
+    so that every iteration is a recording point; an off-by-one in the
+    index moves |z| by a factor ~|z|^2 near escape."""
+    f = frames.make("M2_E20", nx=32)
+    args, c_pix = frames.frame_inputs(f)
+    fields = dict(args.fields, bla_on=0, calc_orbit=1, backshift=backshift)
+    Z, U, sr, si = orc.OracleCtx(fields, args.arrays).cycles(c_pix)
+    assert Z.shape[0] == 3
+    n_checked = 0
+    for i in sample(len(c_pix), 16):
+        if sr[0, i] != 1:
+            continue
+        zk = brute.holo_point(f, c_pix[i], 2, int(si[0, i]) - backshift)
+        assert abs(Z[-1, i] - zk) <= 1e-6 * abs(zk), (i, Z[-1, i], zk)
+        n_checked += 1
+    assert n_checked >= 8
+
+
+def test_calc_orbit_bs_vs_brute_force():
+    """BS calc_orbit: the finish loop restarts from the last-but-one record
+    (every multiple of backshift with BLA off) and iterates with
+    (a + Re Zn_path[1], b + Im Zn_path[1]) (perturbation.py:1778-1787).
+    Y_1 = -B_ref, so that second constant is B - 2 B_ref, not the pixel's
+    B: kept as the reference has it.  Brute force: the true orbit point at
+    the record, then the same finish in mpmath."""
+    import mpmath
+    from fractalshades_amd.mpmath_loop import bs_iterate
+    bsh = 4
+    f = frames.make("BS_ex14", nx=32, max_iter=20000)
+    args, c_pix = frames.frame_inputs(f)
+    fl = args.fields["flavor"]
+    fields = dict(args.fields, bla_on=0, calc_orbit=1, backshift=bsh)
+    Z, U, sr, si = orc.OracleCtx(fields, args.arrays).cycles(c_pix)
+    assert Z.shape[0] == 8
+    Zn1 = args.arrays["Zn_path"].view(np.complex128)[1]
+    n_checked = 0
+    escaped = np.flatnonzero(sr[0] == 1)
+    for i in np.random.default_rng(0).choice(escaped, min(16, escaped.size), replace=False):
+        n = int(si[0, i])
+        # pixels whose own escape the binary64 loop tracks (over ~4000
+        # steps some BS pixels are chaotic: see test_bs_vs_brute_force)
+        nb, zb = brute.bs_escape(f, c_pix[i], fl, 20000, 1e3, prec=f.prec + 128)
+        if nb != n or abs(complex(Z[0, i], Z[1, i]) - zb) > 1e-8 * abs(zb):
+            continue
+        i2 = bsh * (n // bsh - 1)
+        prec = f.prec + 128
+        z2 = brute.bs_point(f, c_pix[i], fl, i2, prec=prec)
+        c = brute.pixel_c(f, c_pix[i])
+        with mpmath.workprec(prec):
+            AA = c.real
+            BB = c.imag + 2 * mpmath.mpf(Zn1.imag)      # b + Im Zn_path[1], Im Zn_path[1] = -B_ref
+            x, y = mpmath.mpf(z2.real), mpmath.mpf(z2.imag)
+            for _ in range(n - bsh - i2):
+                x, y = bs_iterate(fl, x, y, AA, BB)
+            want = complex(x, y)
+        got = complex(Z[6, i], Z[7, i])
+        assert abs(got - want) <= 1e-6 * abs(want), (i, got, want)
+        n_checked += 1
+    assert n_checked >= 6
+
+
+@pytest.mark.parametrize("name", ["BS_ex14", "PBS_ex18"])
+def test_bs_vs_brute_force(name):
+    """BS loop, BLA off, vs direct iteration at the frame's precision: escape
+    counts exact on most pixels and the escaped (x, y) within 1e-8 on most;
+    the rest are chaotic over these 4000-12000-step orbits, where binary64
+    rounding (the reference's as much as this one's) decides."""
+    f = frames.make(name, nx=24, max_iter=20000)
+    args, c_pix = frames.frame_inputs(f)
+    fl = args.fields["flavor"]
+    Z, U, sr, si = orc.OracleCtx(dict(args.fields, bla_on=0), args.arrays).cycles(c_pix)
+    sel = sample(len(c_pix), 12)
+    same_n = close = 0
+    for i in sel:
+        nb, zb = brute.bs_escape(f, c_pix[i], fl, 20000, 1e3, prec=f.prec + 128)
+        same_n += int(nb == si[0, i])
+        close += int(nb == si[0, i] and abs(complex(Z[0, i], Z[1, i]) - zb) <= 1e-8 * abs(zb))
+    assert same_n >= 9 and close >= 6, (same_n, close)
