@@ -48,33 +48,62 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// A lane whose pixel stops goes idle; the results are written and new
+// pixels taken only once REFILL_BATCH lanes of the wave are idle (or all of
+// them).  Writing a pixel's results and starting the next one costs about as
+// much as a few iterations, and under SIMT a wave pays it for every trip on
+// which any lane switches -- with one switch per trip on average when each
+// lane switches on its own.  Batching the switches trades a few idle lanes for
+// that cost.
+constexpr int REFILL_BATCH = 16;
+
 template <class Model>
 __global__ void __launch_bounds__(256) k_tile(const KArgs a) {
     typename Model::State s;
     Counters k = {0, 0, 0, 0, 0};
-    int64_t idx = 0;
-    bool need = true;
+    int64_t idx = -1;
+    int reason = 0;
+    bool active = false;      // iterating a pixel
+    bool pending = false;     // holding a stopped pixel whose results are not written yet
+    bool exhausted = false;   // the cursor has no pixel left for this lane
     for (;;) {
-        if (need) {
-            // lanes that need a pixel take consecutive indices from one atomic
-            const unsigned long long mask = __ballot(1);
-            const int leader = __ffsll((long long)mask) - 1;
-            const unsigned rank = __builtin_amdgcn_mbcnt_hi(
-                (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-            unsigned long long base = 0;
-            if ((int)__lane_id() == leader)
-                base = atomicAdd(a.work, (unsigned long long)__popcll(mask));
-            const unsigned lo = __builtin_amdgcn_readlane((unsigned)base, leader);
-            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(base >> 32), leader);
-            idx = (int64_t)(((unsigned long long)hi << 32) | lo) + rank;
-            if (idx >= a.npts) break;
-            Model::init(s, a, idx);
-            need = false;
+        const unsigned long long act = __ballot(active);
+        if (act == 0ull || 64 - __popcll(act) >= REFILL_BATCH) {
+            if (!active) {
+                if (pending) {
+                    Model::finish(s, a, idx, reason);
+                    pending = false;
+                }
+                if (!exhausted) {
+                    // idle lanes take consecutive indices from one atomic
+                    const unsigned long long mask = __ballot(1);
+                    const int leader = __ffsll((long long)mask) - 1;
+                    const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                    unsigned long long base = 0;
+                    if ((int)__lane_id() == leader)
+                        base = atomicAdd(a.work, (unsigned long long)__popcll(mask));
+                    const unsigned lo = __builtin_amdgcn_readlane((unsigned)base, leader);
+                    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(base >> 32), leader);
+                    idx = (int64_t)(((unsigned long long)hi << 32) | lo) + rank;
+                    if (idx < a.npts) {
+                        Model::init(s, a, idx);
+                        active = true;
+                    } else {
+                        exhausted = true;
+                    }
+                }
+            }
+            // every lane idle after the refill: the cursor is exhausted and
+            // every result has been written
+            if (__ballot(active) == 0ull) break;
         }
-        const int reason = Model::step(s, a, k);
-        if (reason >= 0) {
-            Model::finish(s, a, idx, reason);
-            need = true;
+        if (active) {
+            reason = Model::step(s, a, k);
+            if (reason >= 0) {
+                active = false;
+                pending = true;
+            }
         }
     }
     // all lanes have left the loop: one atomic per wave per counter
