@@ -30,7 +30,7 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector / matrix peak
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
 # on a plain 256 MiB copy (`bench.py --calib-copy`): FETCH x2.0, WRITE x1.0.
 # Summary and commands: profiles/k_tile_C2_2048sq_1e-300.md.
-MEASURED_TRAFFIC = {("C2", 0): 373.2e6}
+MEASURED_TRAFFIC = {("C2", 0): 335.3e6}
 
 
 def calib_copy():
@@ -123,8 +123,12 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
     finally:
         srv.close()
 
-    def lat(rs, size=None):
-        v = np.array([r["latency_ms"] for r in rs if size is None or r["pixels"] == size])
+    for rs in (burst, steady):
+        for r, sp in zip(rs, specs):
+            r["shape"] = (sp.nx, sp.ny)
+
+    def lat(rs, shape=None):
+        v = np.array([r["latency_ms"] for r in rs if shape is None or r["shape"] == shape])
         if v.size == 0:
             return None
         return {"n": int(v.size), "p50_ms": float(np.percentile(v, 50)),
@@ -139,8 +143,8 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
                   "latency": lat(burst)},
         "poisson": {"rate_req_per_s": rate, "span_s": span,
                     "pixel_iters_per_s": sum(r["pixel_iters"] for r in steady) / span,
-                    "latency": lat(steady), "latency_512sq": lat(steady, 512 * 512),
-                    "latency_by_size": {f"{nx}x{ny}": lat(steady, nx * ny) for nx, ny in serving.SIZES}},
+                    "latency": lat(steady), "latency_512sq": lat(steady, (512, 512)),
+                    "latency_by_size": {f"{nx}x{ny}": lat(steady, (nx, ny)) for nx, ny in serving.SIZES}},
     }
     for t in tiles:
         t.close()
