@@ -35,6 +35,15 @@ struct TablesX {
     const void *Ax, *Bx;       // [bla_len] BLA coefficients
 };
 
+// Layout of the reference tables staged in LDS for one launch (byte
+// offsets; -1 = not staged).  K orbit points [0, K) cover every index the
+// loop reads (K = min(L, div_iter, order)); dZndz's wrapped value [L] sits
+// at [K]; KB BLA entries cover every jump starting below K.
+struct LdsPlan {
+    int32_t K, KB, bytes, bla_dim;
+    int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, r, M;
+};
+
 // One launch of the tile kernel.
 struct KArgs {
     Tables t;
@@ -54,6 +63,7 @@ struct KArgs {
     int64_t ld;
     int32_t nz; int32_t max_blocks;   // grid cap: 0 fill the GPU, > 0 blocks, -k 1/k of the GPU
     int32_t calc_orbit, backshift;    // orbit point at exit - backshift in the last field(s)
+    LdsPlan lds;                      // lds.bytes == 0: tables read from global memory
     // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
     // rebase, bla lookups)
     unsigned long long *work;
@@ -62,9 +72,11 @@ struct KArgs {
 struct LaunchCfg { int kind, degree, flavor, xr, precision, orbit; };
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
-// the calc_orbit instantiations (fsg_orbit.hip)
+// the calc_orbit instantiations (fsg_orbit.hip) and the LDS-staged ones
+// (fsg_lds.hip, 1024-lane blocks)
 typedef void (*tile_fn)(const KArgs);
 tile_fn select_tile_orbit(const LaunchCfg &cfg);
+tile_fn select_tile_lds(const LaunchCfg &cfg);
 // radii_only: M already holds the coefficients (shared with another
 // context of the same orbit); write the radii r only
 hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,
@@ -72,7 +84,9 @@ hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, in
                             hipStream_t stream, bool radii_only = false);
 int tile_kernel_blocks(const LaunchCfg &cfg);
 int grid_fill_blocks(const void *kernel);          // blocks filling the GPU at full occupancy
-int launch_blocks(int fill, const KArgs &a);       // capped by max_blocks and by the pixel count
+// plan the LDS staging of a launch's tables (bytes = 0 when they do not fit)
+LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes);
+int launch_blocks(int fill, const KArgs &a, int block = 256);   // capped by max_blocks and the pixel count
 
 // FSG_PREC_FP32X kernels (fsg_fp32x.hip)
 constexpr size_t XS_BYTES = 12;

@@ -16,6 +16,9 @@
 //   BLA: numba_make_BLA* / combine_BLA* / ref_BLA_get, perturbation.py:1819-2170
 #include "tile_models.hpp"
 
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <unordered_map>
 
@@ -197,16 +200,70 @@ int tile_kernel_blocks(const LaunchCfg &cfg) {
     return grid_fill_blocks(reinterpret_cast<const void *>(f));
 }
 
-int launch_blocks(int fill, const KArgs &a) {
+int launch_blocks(int fill, const KArgs &a, int block) {
     int blocks = fill;
     if (a.max_blocks > 0 && a.max_blocks < blocks) blocks = a.max_blocks;
     if (a.max_blocks < 0) blocks = (fill + (-a.max_blocks) - 1) / (-a.max_blocks);   // a share of the GPU
-    const int64_t need = (a.npts + 255) / 256;
+    const int64_t need = (a.npts + block - 1) / block;
     if (need < blocks) blocks = (int)(need > 0 ? need : 1);
     return blocks;
 }
 
+static int max_lds_bytes() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 0;
+        cached = v;
+    }
+    return cached;
+}
+
+// Byte offsets of every table the loop reads, staged in LDS; bytes = 0 when
+// they do not fit in max_bytes.
+LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
+    LdsPlan p;
+    std::memset(&p, 0xff, sizeof(p));               // every offset -1
+    int64_t K = std::min<int64_t>(t.L, std::min<int64_t>((int64_t)t.div_iter + 1, t.order));
+    K = std::max<int64_t>(K, std::min<int64_t>(t.L, 2));   // Zn[1] (orbit finish)
+    const int64_t bla_len = 2 * (int64_t)(t.L >> 3);
+    const int64_t KB = t.bla_on ? std::min<int64_t>(bla_len, K / 4 + 2) : 0;
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) { int64_t o = off; off += (bytes + 15) & ~(int64_t)15; return (int32_t)o; };
+    p.zn = take(16 * K);
+    if (t.dc) p.dc = take(16 * K);
+    if (t.dc_e) p.dc_e = take(4 * K);
+    if (t.dz) p.dz = take(16 * (K + 1));
+    if (t.dz_e) p.dz_e = take(4 * (K + 1));
+    for (int q = 0; q < 4; q++) {
+        if (t.bs[q]) p.bs[q] = take(8 * K);
+        if (t.bs_e[q]) p.bs_e[q] = take(4 * K);
+    }
+    if (t.xr_pos) p.xr_pos = take(4 * K);
+    if (KB > 0) { p.r = take(8 * KB); p.M = take(8 * bla_dim * KB); }
+    p.K = (int32_t)K; p.KB = (int32_t)KB; p.bla_dim = bla_dim;
+    p.bytes = (off <= max_bytes && off < INT32_MAX) ? (int32_t)off : 0;
+    return p;
+}
+
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
+    static const bool no_lds = [] { const char *e = std::getenv("FSG_TABLES"); return e && !std::strcmp(e, "global"); }();
+    if (!cfg.orbit && !no_lds) {
+        KArgs al = a;
+        al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());
+        tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg) : nullptr;
+        int per_cu = 0, dev = 0, ncu = 0;
+        if (f && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), 1024,
+                                                         (size_t)al.lds.bytes) == hipSuccess && per_cu > 0) {
+            hipLaunchKernelGGL(f, dim3(launch_blocks(ncu * per_cu, al, 1024)), dim3(1024), (size_t)al.lds.bytes,
+                               stream, al);
+            return hipGetLastError();
+        }
+    }
     tile_fn f = select_tile(cfg);
     if (!f) return hipErrorInvalidValue;
     const int fill = grid_fill_blocks(reinterpret_cast<const void *>(f));
