@@ -137,8 +137,7 @@ template <int KIND, int DEG>
 struct HoloF32 {
     struct State { xs z, d, h, c; int32_t w, n; bool dyn, nul; };
 
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        const double px = a.pix[2 * i], py = a.pix[2 * i + 1];
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
         // c = dx * (lin_mat . pix) + drift, assembled in binary64 mantissas
         const double x1 = a.lin[0] * px + a.lin[1] * py;
         const double y1 = a.lin[2] * px + a.lin[3] * py;

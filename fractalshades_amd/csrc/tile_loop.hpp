@@ -89,7 +89,7 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
                     const unsigned hi = __builtin_amdgcn_readlane((unsigned)(base >> 32), leader);
                     idx = (int64_t)(((unsigned long long)hi << 32) | lo) + rank;
                     if (idx < a.npts) {
-                        Model::init(s, a, idx);
+                        Model::init(s, a, a.pix[2 * idx], a.pix[2 * idx + 1]);
                         active = true;
                     } else {
                         exhausted = true;

@@ -116,8 +116,8 @@ template <int KIND, int DEG, bool ORB = false>
 struct HoloStd {
     struct State { cplx z, d, h, c; int32_t w, n; bool dyn, nul; OrbitRec<ORB> orb; };
 
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        s.c = to_std(c_from_pix(a, a.pix[2 * i], a.pix[2 * i + 1]));
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
+        s.c = to_std(c_from_pix(a, px, py));
         s.z = zc(); s.d = zc(); s.h = zc();
         s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
         s.orb.reset(a.backshift);
@@ -211,8 +211,8 @@ struct HoloXr {
         return xl_of(t.Zn[i]);
     }
 
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        s.c = c_from_pix_xl(a, a.pix[2 * i], a.pix[2 * i + 1]);
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
+        s.c = c_from_pix_xl(a, px, py);
         s.z = xl_zero(); s.d = xl_zero(); s.h = xl_zero();
         s.zs = zc();
         s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
@@ -308,9 +308,9 @@ template <int FL, bool ORB = false>
 struct BsStd {
     struct State { double x, y, P[4], a, b; int32_t w, n; bool dyn; OrbitRec<ORB> orb; };
 
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        xc cx = renorm_c(a.dx_m * (a.lin[0] * a.pix[2 * i] + a.lin[1] * a.pix[2 * i + 1]),
-                         a.dx_m * (a.lin[2] * a.pix[2 * i] + a.lin[3] * a.pix[2 * i + 1]), a.dx_e);
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
+        xc cx = renorm_c(a.dx_m * (a.lin[0] * px + a.lin[1] * py),
+                         a.dx_m * (a.lin[2] * px + a.lin[3] * py), a.dx_e);
         s.a = to_std(mkf(cx.re, cx.e) + mkf(a.drift_m[0], a.drift_e[0]));
         s.b = to_std(mkf(cx.im, cx.e) + mkf(a.drift_m[1], a.drift_e[1]));
         s.x = 0.; s.y = 0.; s.P[0] = s.P[1] = s.P[2] = s.P[3] = 0.;
@@ -394,8 +394,7 @@ struct BsXr {
         else { const cplx R = t.Zn[i]; X = xlf_of(R.re, 0); Y = xlf_of(R.im, 0); }
     }
 
-    __device__ static void init(State &s, const KArgs &a, int64_t i) {
-        const double px = a.pix[2 * i], py = a.pix[2 * i + 1];
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
         s.a = xlf_of(a.dx_m * (a.lin[0] * px + a.lin[1] * py), a.dx_e) + xlf_of(a.drift_m[0], a.drift_e[0]);
         s.b = xlf_of(a.dx_m * (a.lin[2] * px + a.lin[3] * py), a.dx_e) + xlf_of(a.drift_m[1], a.drift_e[1]);
         const xlf z0 = xlf_zero();
