@@ -91,19 +91,23 @@ def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
     }
 
 
-def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
+def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
     """The request stream: seeded requests of mixed size / width / max_iter
     on eight centres (fractalshades_amd.serving), first all at once (the
     makespan gives the capacity), then with Poisson arrivals at `rate_frac`
     of that capacity (latency under sustained load)."""
     from fractalshades_amd import serving
+    from fractalshades_amd.postproc import post_desc
     t0 = time.perf_counter()
     centres = serving.serve_centres()
     specs = serving.serve_requests(n_requests, len(centres), seed=seed)
     bases = {}
     reqs = []
+    posts = [] if layers else None
     for sp in specs:
         f, cia, c_pix = serving.build_request(sp, centres, precision)
+        if layers:
+            posts.append(post_desc(f, "serve"))
         key = (sp.centre, cia.fields["xr_mode"])
         if key not in bases:
             bases[key] = cia.ctx            # resident orbit + BLA coefficients of this centre
@@ -112,14 +116,14 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
         _ = cia.ctx
         reqs.append((cia, c_pix))
     prep_s = time.perf_counter() - t0
-    tiles = serving.pinned_tiles(reqs)
+    tiles = serving.layer_tiles(reqs) if layers else serving.pinned_tiles(reqs)
     srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
     try:
-        burst = serving.run_stream(srv, reqs, None, tiles=tiles)
+        burst = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts)
         t_first = min(r["t_submit_ms"] for r in burst)
         makespan_s = (max(r["t_done_ms"] for r in burst) - t_first) * 1e-3
         rate = rate_frac * n_requests / makespan_s
-        steady = serving.run_stream(srv, reqs, rate, seed=seed + 1, tiles=tiles)
+        steady = serving.run_stream(srv, reqs, rate, seed=seed + 1, tiles=tiles, posts=posts)
     finally:
         srv.close()
 
@@ -138,6 +142,8 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0):
     span = (max(r["t_done_ms"] for r in steady) - min(r["t_submit_ms"] for r in steady)) * 1e-3
     out = {
         "requests": n_requests, "precision": precision, "host_prep_s": round(prep_s, 1),
+        "returns": "nu/dem float32 layers + stop reason/iteration (13 B/px)" if layers
+                   else "raw Z/U/stop reason/iteration (57 B/px)",
         "pixels": int(sum(r["pixels"] for r in burst)), "pixel_iters": int(pix_iters),
         "burst": {"makespan_s": makespan_s, "pixel_iters_per_s": pix_iters / makespan_s,
                   "latency": lat(burst)},
@@ -312,11 +318,12 @@ def main():
         },
     }
     if args.serve > 0:
-        serve = {"fp64": serving_leg(args.serve, "fp64")}
+        serve = {"fp64_layers": serving_leg(args.serve, "fp64", layers=True),
+                 "fp64_raw": serving_leg(args.serve, "fp64")}
         if not args.no_fp32x:
-            serve["fp32x"] = serving_leg(args.serve, "fp32x")
+            serve["fp32x_layers"] = serving_leg(args.serve, "fp32x", layers=True)
         out["config"]["serving"] = serve
-        lat512 = serve["fp64"]["poisson"]["latency_512sq"]
+        lat512 = serve["fp64_layers"]["poisson"]["latency_512sq"]
         if lat512:
             out["tile_latency_512sq_ms"] = {"p50": lat512["p50_ms"], "p99": lat512["p99_ms"]}
     if not args.no_cpu:

@@ -23,6 +23,7 @@ EXPORTS = (
     "fsg_ctx_read_table", "fsg_ctx_bla_shape", "fsg_last_error",
     "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
     "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
+    "fsg_postproc_device", "fsg_server_submit_layers",
 )
 FSG_PENDING = 2
 
@@ -75,6 +76,15 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PostDesc(ctypes.Structure):
+    """fsg_post_desc (include/fsgpu.h)."""
+    _fields_ = [
+        ("holomorphic", ctypes.c_int32), ("dzndc_row", ctypes.c_int32),
+        ("d", ctypes.c_double), ("a_d", ctypes.c_double), ("M", ctypes.c_double),
+        ("floor_iter", ctypes.c_double), ("px_snap", ctypes.c_double),
+    ]
 
 
 class ServerOpts(ctypes.Structure):
@@ -133,6 +143,9 @@ def load(path=None):
             "fsg_server_submit": ([vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
                                   ctypes.c_int),
             "fsg_server_wait": ([vp, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)], ctypes.c_int),
+            "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp], ctypes.c_int),
+            "fsg_server_submit_layers": ([vp, vp, vp, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp,
+                                          ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "fsg_host_alloc": ([ctypes.c_size_t], vp),
             "fsg_host_free": ([vp], ctypes.c_int),
         }

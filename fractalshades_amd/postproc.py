@@ -11,7 +11,8 @@ count (potential) and distance estimate (DEM).
 
 Computed in float64 (the reference's default post-processing dtype is
 float32, settings.py:82; the parity checks need float64).  Also: the error
-report of the binary32 form against the binary64 drop-in.
+report of the binary32 form against the binary64 drop-in, and the
+descriptor of the same two layers computed on the GPU (fsg_post_desc).
 """
 import numpy as np
 
@@ -96,3 +97,20 @@ def error_report(test, ref, holomorphic=True, d=2, M=1000.):
             out["dem_rel_err_p99"] = float(np.quantile(rel, 0.99))
             out["dem_rel_err_max"] = float(rel.max())
     return out
+
+
+def post_desc(f, calc_name, floor_iter=0., px_snap=None):
+    """fsg_post_desc of a calculation: potential parameters of the model
+    (potential_d, potential_a_d, potential_M, as Continuous_iter_pp reads
+    them, postproc.py:320-347) and the row of dz/dc (dX/da for the burning
+    ship) in its raw Z array."""
+    from ._native import PostDesc
+    state = f._calc_data[calc_name]["state"]
+    names = list(state.codes[0])
+    holo = bool(getattr(f, "holomorphic", True))
+    row = names.index("dzndc") if holo and "dzndc" in names else (
+        names.index("dxnda") if (not holo and "dxnda" in names) else -1)
+    d = float(getattr(f, "potential_d", 2))
+    a_d = float(getattr(f, "potential_a_d", 1.))
+    M = float(getattr(state, "potential_M", getattr(f, "potential_M_cutoff", 1000.)))
+    return PostDesc(int(holo), int(row), d, a_d, M, float(floor_iter), float(px_snap or 0.))

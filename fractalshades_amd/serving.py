@@ -65,6 +65,30 @@ class PinnedTile:
             pass
 
 
+class LayerTile:
+    """c_pix in, the two post-processing layers (float32 nu, dem) with
+    stop_reason and stop_iter out, in pinned memory."""
+
+    def __init__(self, n):
+        lib = load()
+        specs = [("c_pix", (n,), np.complex128), ("nu", (n,), np.float32), ("dem", (n,), np.float32),
+                 ("stop_reason", (1, n), np.int8), ("stop_iter", (1, n), np.int32)]
+        self._ptrs = []
+        for name, shape, dt in specs:
+            nbytes = int(np.prod(shape)) * np.dtype(dt).itemsize
+            p = lib.fsg_host_alloc(max(nbytes, 1))
+            if not p:
+                self.close()
+                raise FsgError("fsg_host_alloc failed")
+            self._ptrs.append(p)
+            buf = (ctypes.c_char * max(nbytes, 1)).from_address(p)
+            setattr(self, name, np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape))
+        self.n = n
+
+    close = PinnedTile.close
+    __del__ = PinnedTile.__del__
+
+
 class TileServer:
     """One native serving loop (fsg_server_create)."""
 
@@ -100,6 +124,18 @@ class TileServer:
                                        tile.stop_iter.ctypes.data, ctypes.byref(t)))
         with self._lock:
             self._keep[t.value] = (ctx, tile)   # alive until waited for
+        return t.value
+
+    def submit_layers(self, ctx, tile, post):
+        """Queue a request returning the nu / dem layers (LayerTile) instead
+        of the raw fields; post is a PostDesc (postproc.post_desc)."""
+        t = ctypes.c_uint64()
+        check(load().fsg_server_submit_layers(
+            self._h, ctx.handle, tile.c_pix.ctypes.data, tile.c_pix.shape[0], ctypes.byref(post),
+            tile.nu.ctypes.data, tile.dem.ctypes.data, tile.stop_reason.ctypes.data,
+            tile.stop_iter.ctypes.data, ctypes.byref(t)))
+        with self._lock:
+            self._keep[t.value] = (ctx, tile, post)
         return t.value
 
     def wait(self, ticket, timeout_ms=-1):
@@ -273,24 +309,38 @@ def pinned_tiles(reqs):
     return tiles
 
 
-def run_stream(server, reqs, rate_hz=None, seed=1, tiles=None):
+def layer_tiles(reqs):
+    """One LayerTile per prepared request, input filled in."""
+    tiles = []
+    for cia, c_pix in reqs:
+        t = LayerTile(c_pix.size)
+        t.c_pix[:] = c_pix
+        tiles.append(t)
+    return tiles
+
+
+def run_stream(server, reqs, rate_hz=None, seed=1, tiles=None, posts=None):
     """Submit prepared requests [(cia, c_pix)] with Poisson arrivals at
-    `rate_hz` (all at once when None); wait for all.  Returns per-request
-    dicts: latency_ms (arrival -> results in host memory), queue_ms,
-    pixels, pixel_iters, tile."""
+    `rate_hz` (all at once when None); wait for all.  With `posts` (one
+    PostDesc per request) the requests return the nu / dem layers instead of
+    the raw fields.  Returns per-request dicts: latency_ms (arrival ->
+    results in host memory), queue_ms, pixels, pixel_iters, tile."""
     rng = np.random.default_rng(seed)
     if tiles is None:
-        tiles = pinned_tiles(reqs)
+        tiles = layer_tiles(reqs) if posts is not None else pinned_tiles(reqs)
     gaps = rng.exponential(1. / rate_hz, len(reqs)) if rate_hz else np.zeros(len(reqs))
     arrivals = np.cumsum(gaps)
     tickets = []
     t0 = time.perf_counter()
     c0 = server.clock()
-    for (cia, _), tile, ta in zip(reqs, tiles, arrivals):
+    for k, ((cia, _), tile, ta) in enumerate(zip(reqs, tiles, arrivals)):
         delay = ta - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        tickets.append(server.submit(cia.ctx, tile))
+        if posts is not None:
+            tickets.append(server.submit_layers(cia.ctx, tile, posts[k]))
+        else:
+            tickets.append(server.submit(cia.ctx, tile))
     out = []
     for (cia, _), tile, tk, ta in zip(reqs, tiles, tickets, arrivals):
         info = server.wait(tk)

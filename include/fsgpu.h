@@ -189,6 +189,33 @@ int fsg_ctx_bla_shape(fsg_ctx *ctx, int64_t *bla_len, int32_t *stages_bla);
 const char *fsg_last_error(void);
 
 /*
+ * ---- Post-processing layers --------------------------------------------------
+ * The continuous iteration count and the distance estimate of each pixel,
+ * from the raw results while they are on the device: Continuous_iter_pp with
+ * kind "infinity" (postproc.py:352-406, 1001-1009) and DEM_pp
+ * (postproc.py:684-731), computed in binary64 and stored as float32 (the
+ * reference's default post-processing dtype, settings.py:82).  Pixels whose
+ * stop reason is not divergence get NaN.  The reference computes these on
+ * the host from the copied Z array; here 8 bytes per pixel leave the device
+ * instead of the raw fields.
+ */
+typedef struct fsg_post_desc {
+    int32_t holomorphic;     /* 1: Z rows are complex128; 0: float64 (xn, yn, Jacobian) */
+    int32_t dzndc_row;       /* row of dz/dc (holomorphic) or of dX/da (burning ship) */
+    double d;                /* potential_d (the model's degree) */
+    double a_d;              /* potential_a_d */
+    double M;                /* potential_M (the escape radius) */
+    double floor_iter;       /* Continuous_iter_pp floor_iter */
+    double px_snap;          /* DEM_pp px_snap; <= 0 for none */
+} fsg_post_desc;
+
+/* Enqueue the layers of n pixels (Z rows of stride ld >= n) on stream;
+ * nu or dem may be null to skip that layer. */
+int fsg_postproc_device(const fsg_post_desc *pp, const double *d_Z, int64_t n, int64_t ld,
+                        const int32_t *d_stop_iter, const int8_t *d_stop_reason, float *d_nu,
+                        float *d_dem, void *stream);
+
+/*
  * ---- Serving loop ----------------------------------------------------------
  * No counterpart in the reference, which renders the tiles of one frame with
  * a thread pool (mthreading.py:48-68).  A server takes tile requests of mixed
@@ -229,6 +256,12 @@ int fsg_server_clock(fsg_server *srv, double *ms);
  * FSG_E_NOMEM if the request alone exceeds the budget. */
 int fsg_server_submit(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z,
                       int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
+/* The same request returning the post-processing layers (nu, dem: float32
+ * [n], either may be null) with stop_reason and stop_iter instead of the raw
+ * fields: the raw results stay on the device. */
+int fsg_server_submit_layers(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n,
+                             const fsg_post_desc *pp, float *nu, float *dem, int8_t *stop_reason,
+                             int32_t *stop_iter, uint64_t *ticket);
 /* Wait for a request (timeout_ms < 0: no limit).  Returns its status, or
  * FSG_PENDING on timeout; after a return other than FSG_PENDING the ticket
  * is released. */

@@ -139,3 +139,40 @@ def test_shared_context_rejects_other_orbit():
     b, _ = frames.frame_inputs(g)
     with pytest.raises(FsgError):
         _ = b.shared_with(a.ctx).ctx
+
+
+@pytest.mark.parametrize("name,model_kw", [("M2_E20", None), ("BS_ex14", {"flavor": "Burning ship"})])
+def test_server_layers_match_host_postproc(name, model_kw):
+    """fsg_server_submit_layers: the continuous iteration count and the DEM
+    computed on the device from the raw results equal the host formulas of
+    postproc.py (Continuous_iter_pp / DEM_pp restated in numpy) on the raw
+    results of fsg_tile, to float32 resolution; NaN where the pixel did not
+    diverge."""
+    from fractalshades_amd.postproc import continuous_iter, dem, post_desc
+    from fractalshades_amd.serving import LayerTile
+    kw = {"max_iter": 6000} if name.startswith("BS") else {}
+    f = frames.make(name, nx=64, model_kw=model_kw, **kw)
+    a, c = frames.frame_inputs(f)
+    (Z, U, sr, si), _ = run_gpu(a, c)
+    esc = sr[0] == 1
+    assert esc.sum() > 100
+    holo = a.fields["kind"] != 2
+    zn = Z[0] if holo else Z[0] + 1j * Z[1]
+    d = f.potential_d
+    nu_ref = continuous_iter(zn[esc], si[0][esc], d=d, a_d=f.potential_a_d, M=1e3)
+    dem_ref = dem(zn[esc], dzndc=Z[-1][esc]) if holo else dem(zn[esc], jac=Z[2:6, esc])
+    pp = post_desc(f, "t")
+    t = LayerTile(c.size)
+    t.c_pix[:] = c
+    srv = TileServer(lanes=2, chunk_px=1024)
+    try:
+        info = srv.wait(srv.submit_layers(a.ctx, t, pp), 60000)
+    finally:
+        srv.close()
+    assert info["status"] == 0
+    np.testing.assert_array_equal(t.stop_iter, si)
+    np.testing.assert_array_equal(t.stop_reason, sr)
+    assert np.all(np.isnan(t.nu[~esc])) and np.all(np.isnan(t.dem[~esc]))
+    np.testing.assert_allclose(t.nu[esc], nu_ref.astype(np.float32), rtol=2e-7, atol=0)
+    np.testing.assert_allclose(t.dem[esc], dem_ref.astype(np.float32), rtol=2e-7, atol=0)
+    t.close()
