@@ -249,7 +249,9 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
 }
 
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
-    static const bool no_lds = [] { const char *e = std::getenv("FSG_TABLES"); return e && !std::strcmp(e, "global"); }();
+    // FSG_TABLES=global keeps the tables in global memory (for comparisons)
+    const char *tables = std::getenv("FSG_TABLES");
+    const bool no_lds = tables && !std::strcmp(tables, "global");
     if (!cfg.orbit && !no_lds) {
         KArgs al = a;
         al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());

@@ -24,7 +24,11 @@ namespace fsg {
 struct xl { double re, im; int32_t e; };
 struct xlf { double m; int32_t e; };
 
-constexpr int32_t XL_ZERO_E = -(1 << 29);
+// exponent of zero: below every exponent a value reaches (|e| < 2^17 down
+// to 1e-20000), and small enough that the product of up to 31 zeros -- a
+// power of a zero orbit point in the z^d + c kernels takes up to 7 -- still
+// fits in int32
+constexpr int32_t XL_ZERO_E = -(1 << 26);
 
 #define FSG_DEV __device__ __forceinline__
 
@@ -60,7 +64,9 @@ FSG_DEV xlf xlf_zero() { xlf r; r.m = 0.; r.e = XL_ZERO_E; return r; }
 FSG_DEV xc xc_of(xl a) { return mkc(a.re, a.im, a.e == XL_ZERO_E ? 0 : a.e); }
 FSG_DEV xf xf_of(xlf a) { return mkf(a.m, a.e == XL_ZERO_E ? 0 : a.e); }
 
-FSG_DEV int32_t shift_of(int32_t e, int32_t E) { return max(e - E, -2100); }
+// alignment shift of exponent e to E >= e; v_ldexp_f64 takes any int32
+// exponent, and the zero sentinel keeps every difference within int32
+FSG_DEV int32_t shift_of(int32_t e, int32_t E) { return e - E; }
 
 // ---- complex ----
 FSG_DEV xl operator+(xl a, xl b) {
@@ -160,15 +166,23 @@ FSG_DEV xz operator+(xz a, xz b) {
     const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
     return xz_mk(ldexp(a.re, sa) + ldexp(b.re, sb), ldexp(a.im, sa) + ldexp(b.im, sb), E);
 }
+// A product of two values is zero only when a factor is (mantissas stay
+// within 2^+-300, far from underflow), and a zero factor carries the
+// sentinel, so the product's exponent is already below any other: no
+// zero test is needed.  (At most 8 sentinels add up within one iteration --
+// Z^7 times a zero path value -- far from the int32 limit; the state is
+// renormalised after.)
 FSG_DEV xz operator*(xz a, xz b) {
     const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
-    return xz_mk(ac - bd, ad + bc, a.e + b.e);
+    xz r;
+    r.re = ac - bd; r.im = ad + bc; r.e = a.e + b.e;
+    return r;
 }
 FSG_DEV xz operator*(xz a, cplx b) {
     const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
     return xz_mk(ac - bd, ad + bc, a.e);
 }
-FSG_DEV xz muls(double s, xz a) { return xz_mk(s * a.re, s * a.im, a.e); }
+FSG_DEV xz muls(double s, xz a) { xz r; r.re = s * a.re; r.im = s * a.im; r.e = a.e; return r; }   // s != 0
 FSG_DEV xlf abs2(xz a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
 
 // burning-ship helpers on Xrange reals (burning_ship.py:19-60)

@@ -46,7 +46,7 @@ for _name, _e in _EX.items():
 
 # z^d + c: seeded boundary-descent centres (fractalshades_amd.configs)
 MN_TEST = {d: configs.Workload(f"T_MN{d}", "MN", "1e-14", 64, 64, 5000, 40, degree=d,
-                               period_range=(10, 3000), seed=d) for d in (3, 4, 6)}
+                               period_range=(10, 3000), seed=d) for d in (2, 3, 4, 5, 6, 7, 8)}
 _mn_centers = {}
 
 

@@ -142,3 +142,40 @@ def test_calc_orbit_bs_bitexact(flavor, xr):
     got, _ = run_gpu(args, c_pix)
     assert got[0].shape[0] == 8
     assert_same(got, ref)
+
+
+@pytest.mark.parametrize("d", [2, 5, 7, 8])
+@pytest.mark.parametrize("xr", [0, 1])
+def test_mn_other_degrees_bitexact(d, xr):
+    f = frames.mn_frame(d, nx=40)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    print(f"case Mn d={d} xr={xr}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, _ = run_gpu(args, c_pix)
+    assert_same(got, ref)
+
+
+@pytest.mark.parametrize("case", ["M2_E20-1", "M2_divref5-0", "Mn3-1", "BS_ex14-Celtic-1", "PBS_ex18-None-0"])
+def test_tables_global_and_lds_bitexact(case, monkeypatch):
+    """The tables staged in LDS (default when they fit) and read from global
+    memory (FSG_TABLES=global) give the oracle's results bit for bit."""
+    parts = case.split("-")
+    if parts[0] == "Mn3":
+        f = frames.mn_frame(3, nx=40)
+    elif parts[0].startswith("M2"):
+        f = frames.make(parts[0], nx=48)
+    else:
+        fl = None if parts[1] == "None" else parts[1]
+        f = frames.make(parts[0], nx=40, model_kw={"flavor": fl} if fl else None, max_iter=6000)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=int(parts[-1]))
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    for mode in ("lds", "global"):
+        print(f"case tables {case} {mode}", flush=True)
+        if mode == "global":
+            monkeypatch.setenv("FSG_TABLES", "global")
+        else:
+            monkeypatch.delenv("FSG_TABLES", raising=False)
+        got, _ = run_gpu(with_fields(args), c_pix)
+        assert_same(got, ref)
