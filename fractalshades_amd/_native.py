@@ -23,8 +23,9 @@ EXPORTS = (
     "fsg_ctx_read_table", "fsg_ctx_bla_shape", "fsg_last_error",
     "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
     "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
-    "fsg_postproc_device", "fsg_server_submit_layers",
+    "fsg_postproc_device", "fsg_server_submit_layers", "fsg_ref_orbit",
 )
+FSG_E_RANGE = -5
 FSG_PENDING = 2
 
 
@@ -108,6 +109,29 @@ _lib = None
 _lock = threading.Lock()
 
 
+class OrbitDesc(ctypes.Structure):
+    """fsg_orbit_desc (include/fsgpu.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("degree", ctypes.c_int32),
+        ("flavor", ctypes.c_int32), ("need_xr", ctypes.c_int32),
+        ("n_limbs", ctypes.c_int32), ("frac_limbs", ctypes.c_int32),
+        ("c_re_neg", ctypes.c_int32), ("c_im_neg", ctypes.c_int32),
+        ("c_re", ctypes.c_void_p), ("c_im", ctypes.c_void_p),
+        ("max_iter", ctypes.c_int64), ("M", ctypes.c_double),
+        ("n_threads", ctypes.c_int32), ("pad0", ctypes.c_int32),
+    ]
+
+
+class OrbitOut(ctypes.Structure):
+    """fsg_orbit_out (include/fsgpu.h)."""
+    _fields_ = [
+        ("Zn", ctypes.c_void_p), ("xr_cap", ctypes.c_int64),
+        ("xr_index", ctypes.c_void_p), ("xr_mant", ctypes.c_void_p),
+        ("xr_exp", ctypes.c_void_p),
+        ("n_xr", ctypes.c_int64), ("div_iter", ctypes.c_int64),
+    ]
+
+
 def load(path=None):
     """Load libfsgpu.so once; raise FsgError when it is not built."""
     global _lib
@@ -148,6 +172,7 @@ def load(path=None):
                                           ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "fsg_host_alloc": ([ctypes.c_size_t], vp),
             "fsg_host_free": ([vp], ctypes.c_int),
+            "fsg_ref_orbit": ([ctypes.POINTER(OrbitDesc), ctypes.POINTER(OrbitOut)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

@@ -21,7 +21,8 @@ LIB = os.path.join(LIBDIR, "libfsgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["fsg_kernels.hip", "fsg_orbit.hip", "fsg_lds.hip", "fsg_api.hip", "fsg_fp32x.hip", "fsg_serve.hip", "fsg_post.hip"]
+SOURCES = ["fsg_kernels.hip", "fsg_orbit.hip", "fsg_lds.hip", "fsg_api.hip", "fsg_fp32x.hip", "fsg_serve.hip", "fsg_post.hip",
+           "fsg_fixed.cpp"]
 # the binary32 form may fuse multiply-adds; the binary64 drop-in may not
 CONTRACT = {"fsg_fp32x.hip": "-ffp-contract=fast"}
 FLAGS = [

@@ -16,12 +16,22 @@ import mpmath
 import numpy as np
 
 from . import settings
+from . import fixed_orbit
 from . import mpmath_loop as fsFP
 from ._native import FSG_KIND_M2, FSG_KIND_MN, FSG_KIND_BS
 from .perturbation import PerturbationFractal
 from .utils import calc_options
 
 BS_FLAVORS = ("Burning ship", "Perpendicular burning ship", "Shark fin", "Celtic", "Buffalo")
+
+
+def _orbits():
+    """The module that runs reference orbits (settings.orbit_backend)."""
+    if settings.orbit_backend == "native":
+        return fixed_orbit
+    if settings.orbit_backend == "mpmath":
+        return fsFP
+    raise ValueError(f"unknown orbit_backend {settings.orbit_backend!r}")
 
 
 def get_flavor_int(flavor):
@@ -90,7 +100,7 @@ class Perturbation_mandelbrot(_HolomorphicBase):
         self.exponent = 2
 
     def FP_loop(self, ref_orbit_len, c0):
-        return fsFP.holo_orbit(c0, 2, ref_orbit_len, self.M_divergence * 2,
+        return _orbits().holo_orbit(c0, 2, ref_orbit_len, self.M_divergence * 2,
                                self.xr_detect_activated, self.prec)
 
     @calc_options
@@ -142,7 +152,7 @@ class Perturbation_mandelbrot_N(_HolomorphicBase):
         return {"directory": self.directory, "exponent": self.exponent}
 
     def FP_loop(self, ref_orbit_len, c0):
-        return fsFP.holo_orbit(c0, self.exponent, ref_orbit_len, self.M_divergence * 2,
+        return _orbits().holo_orbit(c0, self.exponent, ref_orbit_len, self.M_divergence * 2,
                                self.xr_detect_activated, self.prec)
 
     @calc_options
@@ -197,7 +207,7 @@ class Perturbation_burning_ship(PerturbationFractal):
         return {"directory": self.directory, "flavor": self.flavor}
 
     def FP_loop(self, ref_orbit_len, c0):
-        return fsFP.bs_orbit(c0, self.flavor_int, ref_orbit_len, self.M_divergence * 2,
+        return _orbits().bs_orbit(c0, self.flavor_int, ref_orbit_len, self.M_divergence * 2,
                              self.xr_detect_activated, self.prec)
 
     @calc_options

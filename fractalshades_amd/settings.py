@@ -20,3 +20,7 @@ postproc_dtype: str = "float32"
 # binary64 / extended-range semantics, bit for bit); "fp32x" runs binary32
 # mantissas with an int32 exponent (faster, approximate).
 gpu_precision: str = "fp64"
+
+# Reference orbit loop: "native" iterates in the library's multi-limb fixed
+# point (fsg_ref_orbit); "mpmath" in mpmath at the same precision.
+orbit_backend: str = "native"

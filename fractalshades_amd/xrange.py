@@ -51,7 +51,9 @@ def xr_to_mpf(m, e):
 
 
 def mpf_to_double(x):
-    """float64 nearest to the mpf x, correctly rounded down to subnormals."""
+    """float64 nearest to the mpf x, correctly rounded down to subnormals;
+    a value that rounds to zero keeps its sign (mpfr_get_d, MPFR_RNDN, as
+    the reference's orbit loop reads its iterates, FP_loop.pyx:376-377)."""
     x = mpmath.mpf(x)
     if x == 0:
         return 0.0
@@ -59,5 +61,5 @@ def mpf_to_double(x):
     if e - 1 >= -1022:
         return float(x)
     # subnormal range: round once, on the 2**-1074 grid
-    q = int(mpmath.nint(mpmath.ldexp(x, 1074)))
-    return math.ldexp(q, -1074)
+    q = int(mpmath.nint(mpmath.ldexp(abs(x), 1074)))
+    return math.copysign(math.ldexp(q, -1074), -1. if x < 0 else 1.)

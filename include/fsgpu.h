@@ -44,6 +44,7 @@ extern "C" {
 #define FSG_E_HIP (-2)
 #define FSG_E_NOMEM (-3)
 #define FSG_E_UNSUPPORTED (-4)
+#define FSG_E_RANGE (-5)   /* an output buffer is too small (fsg_ref_orbit) */
 
 /*
  * Everything the reference gathers in cycle_indep_args
@@ -270,6 +271,49 @@ int fsg_server_wait(fsg_server *srv, uint64_t ticket, int32_t timeout_ms, fsg_re
 /* Pinned host memory for request buffers. */
 void *fsg_host_alloc(size_t bytes);
 int fsg_host_free(void *p);
+
+/*
+ * ---- Reference orbit ------------------------------------------------------
+ * The arbitrary-precision orbit of the reference point, from the critical
+ * point 0, rounded to complex128, with an Xrange copy of the iterates below
+ * 1e-300.  Replaces
+ *   perturbation_mandelbrotN_select_FP_loop (mpmath_utils/FP_loop.pyx:274-421)
+ *   perturbation_nonholomorphic_FP_loop     (FP_loop.pyx:1828-1979)
+ * which iterate in MPFR/MPC from decimal seed strings at seed_prec bits.
+ * Here the caller passes c already rounded to its precision, as two
+ * fixed-point values: a sign and a magnitude of n_limbs little-endian 64-bit
+ * limbs, of which the low frac_limbs are the fraction (value = +-magnitude *
+ * 2^(-64 frac_limbs)).  64 * frac_limbs >= prec + 128 keeps each operation's
+ * error far below the reference's half ulp at prec bits; the integer limbs
+ * must hold M^degree (2 M^2 for the burning ship).  Runs on the host.
+ */
+typedef struct fsg_orbit_desc {
+    int32_t kind;          /* FSG_KIND_M2 / FSG_KIND_MN / FSG_KIND_BS */
+    int32_t degree;        /* z^d + c (MN): 2..64 */
+    int32_t flavor;        /* burning ship: 1..5 (burning_ship.py:63-76 order) */
+    int32_t need_xr;       /* keep Xrange copies of iterates below 1e-300 */
+    int32_t n_limbs;       /* limbs per value */
+    int32_t frac_limbs;    /* fraction limbs among them */
+    int32_t c_re_neg, c_im_neg;
+    const uint64_t *c_re;  /* [n_limbs] */
+    const uint64_t *c_im;  /* [n_limbs] */
+    int64_t max_iter;      /* iterates 1..max_iter (FP_loop's max_iter) */
+    double M;              /* escape radius: stop once |Z_i| > M */
+    int32_t n_threads;     /* host threads sharing each product (0: automatic) */
+    int32_t pad0;
+} fsg_orbit_desc;
+
+typedef struct fsg_orbit_out {
+    double *Zn;            /* complex128 [max_iter + 1] (caller's); Zn[0] = 0, 0 past the escape */
+    int64_t xr_cap;        /* capacity of the register arrays */
+    int64_t *xr_index;     /* [xr_cap] iteration of each register entry */
+    double *xr_mant;       /* [xr_cap][2]: holomorphic (re, im) on one exponent; BS (x, y) */
+    int32_t *xr_exp;       /* [xr_cap][2]: holomorphic (e, e); BS (x_e, y_e) */
+    int64_t n_xr;          /* out: entries of the register (FSG_E_RANGE when > xr_cap) */
+    int64_t div_iter;      /* out: first invalid index (escape, or max_iter + 1) */
+} fsg_orbit_out;
+
+int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o);
 
 #ifdef __cplusplus
 }
