@@ -24,6 +24,7 @@ EXPORTS = (
     "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
     "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
     "fsg_postproc_device", "fsg_server_submit_layers", "fsg_ref_orbit",
+    "fsg_ball_period", "fsg_newton_eval",
 )
 FSG_E_RANGE = -5
 FSG_PENDING = 2
@@ -132,6 +133,14 @@ class OrbitOut(ctypes.Structure):
     ]
 
 
+class MpValues(ctypes.Structure):
+    """fsg_mp_values (include/fsgpu.h)."""
+    _fields_ = [
+        ("cap", ctypes.c_int64), ("limbs", ctypes.c_void_p), ("neg", ctypes.c_void_p),
+        ("exp", ctypes.c_void_p), ("n", ctypes.c_int64),
+    ]
+
+
 def load(path=None):
     """Load libfsgpu.so once; raise FsgError when it is not built."""
     global _lib
@@ -173,6 +182,10 @@ def load(path=None):
             "fsg_host_alloc": ([ctypes.c_size_t], vp),
             "fsg_host_free": ([vp], ctypes.c_int),
             "fsg_ref_orbit": ([ctypes.POINTER(OrbitDesc), ctypes.POINTER(OrbitOut)], ctypes.c_int),
+            "fsg_ball_period": ([ctypes.POINTER(OrbitDesc), ctypes.c_double, ctypes.c_int64,
+                                 ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+            "fsg_newton_eval": ([ctypes.POINTER(OrbitDesc), ctypes.c_int64, ctypes.c_int32,
+                                 ctypes.POINTER(MpValues)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

@@ -265,6 +265,7 @@ class Arith {
         e = ex;
     }
     bool overflowed() const { return overflow_; }
+    void flag_overflow() const { overflow_ = true; }
 
   private:
     static void chunk_job(void *self, int t) { static_cast<Arith *>(self)->columns(t); }
@@ -400,12 +401,10 @@ Fx load(const Arith &A, const uint64_t *limbs, int32_t neg) {
     return r;
 }
 
-}  // namespace
-
-extern "C" {
-
-int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
-    if (!d || !o) return set_error(FSG_E_INVALID, "null orbit descriptor");
+// descriptor checks shared by the entry points; extra_bits: integer bits
+// needed beyond M^degree
+int check_desc(const fsg_orbit_desc *d, double extra_bits) {
+    if (!d) return set_error(FSG_E_INVALID, "null orbit descriptor");
     if (d->kind != FSG_KIND_M2 && d->kind != FSG_KIND_MN && d->kind != FSG_KIND_BS)
         return set_error(FSG_E_INVALID, "unknown model kind");
     const bool holo = d->kind != FSG_KIND_BS;
@@ -414,16 +413,17 @@ int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
     if (!holo && (d->flavor < 1 || d->flavor > 5)) return set_error(FSG_E_INVALID, "burning-ship flavor must be 1..5");
     if (d->frac_limbs < 1 || d->n_limbs <= d->frac_limbs || d->n_limbs > (1 << 16))
         return set_error(FSG_E_INVALID, "bad limb counts");
-    if (!d->c_re || !d->c_im || !o->Zn || d->max_iter < 0 || !(d->M > 0.))
+    if (!d->c_re || !d->c_im || d->max_iter < 0 || !(d->M > 0.))
         return set_error(FSG_E_INVALID, "bad orbit arguments");
-    if (o->xr_cap < 0 || (o->xr_cap > 0 && (!o->xr_index || !o->xr_mant || !o->xr_exp)))
-        return set_error(FSG_E_INVALID, "bad Xrange register buffers");
     // the integer part must hold M^d + |c|: the last iterate before the escape test
-    const double int_bits = (holo ? deg : 2) * std::log2(d->M + 2.) + 3.;
+    const double int_bits = (holo ? deg : 2) * std::log2(d->M + 2.) + 3. + extra_bits;
     if (int_bits > 64. * (d->n_limbs - d->frac_limbs))
         return set_error(FSG_E_INVALID, "too few integer limbs for the escape radius");
+    return 0;
+}
 
-    // threads for the products: worth their hand-off above ~2500 bits
+// threads for the products: worth their hand-off above ~2500 bits
+std::unique_ptr<Team> make_team(const fsg_orbit_desc *d) {
     int T = d->n_threads;
     if (T <= 0) {
         const int hw = (int)std::thread::hardware_concurrency();
@@ -432,6 +432,223 @@ int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
     T = std::min(T, 64);
     std::unique_ptr<Team> team;
     if (T > 1) team.reset(new Team(T));
+    return team;
+}
+
+// ---- derivatives: block floating point -------------------------------------
+// The derivatives of the orbit with respect to c grow without bound, so they
+// are kept as K fixed-point mantissas sharing one binary exponent:
+// value_k = v[k] * 2^e, with the largest |v[k]| in [1, 2) (all zero: e = 0).
+// A product with a fixed-point iterate is rounded on the fixed-point grid,
+// so the relative error of a derivative stays near 2^-(64 nf) times the
+// inverse size of the iterate it was multiplied by.
+struct Blk {
+    std::vector<Fx> v;
+    int64_t e = 0;
+};
+
+// shift |a| right by s >= 0 bits (truncating)
+void shr(const Arith &A, Fx &a, int64_t s) {
+    const int n = A.n;
+    if (s >= 64LL * n) { std::fill(a.m.begin(), a.m.end(), 0); a.neg = false; return; }
+    const int q = (int)(s >> 6), b = (int)(s & 63);
+    for (int k = 0; k < n; k++) {
+        const int src = k + q;
+        u64 w = src < n ? a.m[src] >> b : 0;
+        if (b && src + 1 < n) w |= a.m[src + 1] << (64 - b);
+        a.m[k] = w;
+    }
+    if (Arith::is_zero(a)) a.neg = false;
+}
+// shift |a| left by s >= 0 bits (the caller keeps it within the limbs)
+void shl(const Arith &A, Fx &a, int64_t s) {
+    const int n = A.n;
+    const int q = (int)(s >> 6), b = (int)(s & 63);
+    for (int k = n - 1; k >= 0; k--) {
+        const int src = k - q;
+        u64 w = src >= 0 ? a.m[src] << b : 0;
+        if (b && src - 1 >= 0) w |= a.m[src - 1] >> (64 - b);
+        a.m[k] = w;
+    }
+}
+// bring the largest mantissa to [1, 2)
+void normalise(const Arith &A, Blk &x) {
+    int top = -1;
+    for (const Fx &f : x.v) top = std::max(top, A.msb(f));
+    if (top < 0) { x.e = 0; for (Fx &f : x.v) f.neg = false; return; }
+    const int64_t s = (int64_t)top - 64LL * A.nf;
+    if (s > 0) for (Fx &f : x.v) shr(A, f, s);
+    else if (s < 0) for (Fx &f : x.v) shl(A, f, -s);
+    x.e += s;
+}
+bool blk_zero(const Blk &x) {
+    for (const Fx &f : x.v) if (!Arith::is_zero(f)) return false;
+    return true;
+}
+// x += y (same K), aligned on the larger exponent, then normalised
+void blk_add(const Arith &A, Blk &x, const Blk &y, Blk &tmp) {
+    if (blk_zero(y)) return;
+    if (blk_zero(x)) { x = y; return; }
+    tmp = y;
+    if (x.e >= tmp.e) { for (Fx &f : tmp.v) shr(A, f, x.e - tmp.e); }
+    else { for (Fx &f : x.v) shr(A, f, tmp.e - x.e); x.e = tmp.e; }
+    for (size_t k = 0; k < x.v.size(); k++) A.add(x.v[k], x.v[k], tmp.v[k]);
+    normalise(A, x);
+}
+// |a| * k (exact), k small
+void mul_small(const Arith &A, Fx &a, u64 k) {
+    u64 carry = 0;
+    for (int i = 0; i < A.n; i++) {
+        const u128 t = (u128)a.m[i] * k + carry;
+        a.m[i] = (u64)t;
+        carry = (u64)(t >> 64);
+    }
+    if (carry) A.flag_overflow();
+}
+
+// binary64 mantissa with a wide exponent, for the ball-method test
+struct XE { double m; int64_t e; };
+XE xe(double m, int64_t e) {
+    if (m == 0.) return {0., 0};
+    int k;
+    const double f = std::frexp(m, &k);
+    return {f, e + k};
+}
+XE xe_of(const Arith &A, const Fx &a, int64_t e) {
+    double m;
+    int32_t k;
+    A.to_xr(a, m, k);
+    return xe(m, (int64_t)k + e);
+}
+XE xe_mul(XE a, XE b) { return xe(a.m * b.m, a.e + b.e); }
+XE xe_div(XE a, XE b) { return xe(a.m / b.m, a.e - b.e); }
+XE xe_add(XE a, XE b) {
+    if (a.m == 0.) return b;
+    if (b.m == 0.) return a;
+    const int64_t E = std::max(a.e, b.e);
+    const auto sh = [](double m, int64_t s) { return s < -2000 ? 0. : std::ldexp(m, (int)s); };
+    return xe(sh(a.m, a.e - E) + sh(b.m, b.e - E), E);
+}
+XE xe_neg(XE a) { a.m = -a.m; return a; }
+XE xe_hypot(XE a, XE b) {
+    if (a.m == 0.) return {std::fabs(b.m), b.e};
+    if (b.m == 0.) return {std::fabs(a.m), a.e};
+    const int64_t E = std::max(a.e, b.e);
+    const auto sh = [](double m, int64_t s) { return s < -2000 ? 0. : std::ldexp(m, (int)s); };
+    return xe(std::hypot(sh(a.m, a.e - E), sh(b.m, b.e - E)), E);
+}
+bool xe_lt1(XE a) { return a.m == 0. || a.e <= 0; }   // |a| < 1 (m in [0.5, 1))
+double xe_double(XE a) { return a.e < -1100 ? 0. : std::ldexp(a.m, (int)std::min<int64_t>(a.e, 1100)); }
+
+// One step of the holomorphic derivative, dz <- d z^(d-1) dz + 1
+// (FP_loop.pyx iter_Mn_deriv, with k = 1)
+struct HoloDeriv {
+    Blk dz, one, tmp;
+    Cx zp, prod;
+    explicit HoloDeriv(const Arith &A) {
+        dz.v = {A.make(), A.make()};
+        one.v = {A.make(), A.make()};
+        one.v[0].m[A.nf] = 1;         // 1 = 2^(64 nf) on the fixed-point grid
+        tmp.v = {A.make(), A.make()};
+        zp.re = A.make(); zp.im = A.make(); prod.re = A.make(); prod.im = A.make();
+    }
+    void step(Arith &A, const Cx &z, int deg, Work &w) {
+        if (!blk_zero(dz)) {
+            const Cx *base = &z;
+            if (deg > 2) { cpow(A, zp, z, deg - 1, w); base = &zp; }
+            Cx m;
+            m.re = dz.v[0]; m.im = dz.v[1];
+            cmul(A, prod, *base, m, w.t1, w.t2, w.t3, w.k1);
+            dz.v[0] = prod.re; dz.v[1] = prod.im;
+            mul_small(A, dz.v[0], (u64)deg);
+            mul_small(A, dz.v[1], (u64)deg);
+            normalise(A, dz);
+        }
+        blk_add(A, dz, one, tmp);
+    }
+};
+
+// Burning-ship Jacobian of (x, y) (burning_ship.py:440-532), entries
+// (dfx/dx, dfx/dy, dfy/dx, dfy/dy) as fixed-point values
+void bs_jac(const Arith &A, int flavor, const Fx &x, const Fx &y, Fx J[4]) {
+    const auto sgn_neg = [](const Fx &v) { return v.neg; };    // sgn(v) = -1 iff v < 0
+    Fx ax = x, ay = y;
+    ax.neg = false; ay.neg = false;
+    // q = sgn(x^2 - y^2) = sgn((x + y)(x - y)), from the magnitudes
+    const int c = A.cmp_mag(x, y);
+    const bool q_neg = c < 0;
+    const auto twice_signed = [&](Fx &r, const Fx &v, bool flip) {
+        A.twice(r, v);
+        if (flip && !Arith::is_zero(r)) r.neg = !r.neg;
+    };
+    switch (flavor) {
+    case 1:
+        twice_signed(J[0], x, false); twice_signed(J[1], y, true);
+        twice_signed(J[2], ay, sgn_neg(x)); twice_signed(J[3], ax, sgn_neg(y));
+        break;
+    case 2:
+        twice_signed(J[0], x, false); twice_signed(J[1], y, true);
+        twice_signed(J[2], ay, false); twice_signed(J[3], x, sgn_neg(y));
+        break;
+    case 3:
+        twice_signed(J[0], x, false); twice_signed(J[1], ay, true);
+        twice_signed(J[2], y, false); twice_signed(J[3], x, false);
+        break;
+    case 4:
+        twice_signed(J[0], x, q_neg); twice_signed(J[1], y, !q_neg);
+        twice_signed(J[2], y, false); twice_signed(J[3], x, false);
+        break;
+    default:
+        twice_signed(J[0], x, q_neg); twice_signed(J[1], y, !q_neg);
+        twice_signed(J[2], ay, sgn_neg(x)); twice_signed(J[3], ax, sgn_neg(y));
+        break;
+    }
+}
+
+// D <- J D + diag(1, -1), D = d(x, y)/d(a, b) of the burning ship
+struct BsDeriv {
+    Blk D, one, tmp;
+    Fx J[4], p, q;
+    explicit BsDeriv(const Arith &A) {
+        D.v = {A.make(), A.make(), A.make(), A.make()};
+        one.v = {A.make(), A.make(), A.make(), A.make()};
+        one.v[0].m[A.nf] = 1;
+        one.v[3].m[A.nf] = 1;
+        one.v[3].neg = true;
+        tmp.v = D.v;
+        for (Fx &f : J) f = A.make();
+        p = A.make(); q = A.make();
+    }
+    void step(Arith &A, int flavor, const Fx &x, const Fx &y) {
+        if (!blk_zero(D)) {
+            bs_jac(A, flavor, x, y, J);
+            Blk r = D;
+            for (int row = 0; row < 2; row++)
+                for (int col = 0; col < 2; col++) {
+                    A.mul(p, J[2 * row], D.v[col]);
+                    A.mul(q, J[2 * row + 1], D.v[2 + col]);
+                    A.add(r.v[2 * row + col], p, q);
+                }
+            D = r;
+            normalise(A, D);
+        }
+        blk_add(A, D, one, tmp);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
+    int rc = check_desc(d, 0.);
+    if (rc) return rc;
+    if (!o || !o->Zn) return set_error(FSG_E_INVALID, "bad orbit output");
+    if (o->xr_cap < 0 || (o->xr_cap > 0 && (!o->xr_index || !o->xr_mant || !o->xr_exp)))
+        return set_error(FSG_E_INVALID, "bad Xrange register buffers");
+    const bool holo = d->kind != FSG_KIND_BS;
+    const int deg = d->kind == FSG_KIND_M2 ? 2 : d->degree;
+    std::unique_ptr<Team> team = make_team(d);
     Arith A(d->n_limbs, d->frac_limbs, team.get());
     Work w(A);
     const Fx cre = load(A, d->c_re, d->c_re_neg), cim = load(A, d->c_im, d->c_im_neg);
@@ -494,6 +711,106 @@ int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
     o->div_iter = i;
     o->n_xr = n_xr;
     if (n_xr > o->xr_cap) return set_error(FSG_E_RANGE, "Xrange register larger than xr_cap");
+    return 0;
+}
+
+
+int fsg_ball_period(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int64_t *order) {
+    int rc = check_desc(d, 8.);
+    if (rc) return rc;
+    if (!order || !(px_mant > 0.) || !std::isfinite(px_mant))
+        return set_error(FSG_E_INVALID, "bad ball-method arguments");
+    const bool holo = d->kind != FSG_KIND_BS;
+    const int deg = d->kind == FSG_KIND_M2 ? 2 : d->degree;
+    std::unique_ptr<Team> team = make_team(d);
+    Arith A(d->n_limbs, d->frac_limbs, team.get());
+    Work w(A);
+    const Fx cre = load(A, d->c_re, d->c_re_neg), cim = load(A, d->c_im, d->c_im_neg);
+    const XE px = xe(px_mant, px_exp);
+    *order = -1;
+    if (holo) {
+        Cx z, zn;
+        z.re = A.make(); z.im = A.make(); zn.re = A.make(); zn.im = A.make();
+        HoloDeriv D(A);
+        for (int64_t i = 1; i <= d->max_iter; i++) {
+            D.step(A, z, deg, w);
+            cpow(A, zn, z, deg, w);
+            A.add(z.re, zn.re, cre);
+            A.add(z.im, zn.im, cim);
+            if (A.overflowed()) return set_error(FSG_E_INVALID, "overflow: escape radius too large");
+            if (std::hypot(A.to_double(z.re), A.to_double(z.im)) > d->M) break;
+            const XE az = xe_hypot(xe_of(A, z.re, 0), xe_of(A, z.im, 0));
+            const XE adz = xe_hypot(xe_of(A, D.dz.v[0], D.dz.e), xe_of(A, D.dz.v[1], D.dz.e));
+            if (xe_lt1(xe_div(xe_div(az, adz), px))) { *order = i; break; }
+        }
+    } else {
+        Fx x = A.make(), y = A.make();
+        BsDeriv D(A);
+        for (int64_t i = 1; i <= d->max_iter; i++) {
+            D.step(A, d->flavor, x, y);
+            bs_step(A, d->flavor, x, y, cre, cim, w);
+            if (A.overflowed()) return set_error(FSG_E_INVALID, "overflow: escape radius too large");
+            if (std::hypot(A.to_double(x), A.to_double(y)) > d->M) break;
+            XE m[4];
+            for (int k = 0; k < 4; k++) m[k] = xe_of(A, D.D.v[k], D.D.e);
+            const XE det = xe_add(xe_mul(m[0], m[3]), xe_neg(xe_mul(m[2], m[1])));
+            if (det.m == 0.) continue;
+            const XE X = xe_of(A, x, 0), Y = xe_of(A, y, 0);
+            const XE rx = xe_div(xe_div(xe_add(xe_mul(m[3], X), xe_neg(xe_mul(m[1], Y))), det), px);
+            const XE ry = xe_div(xe_div(xe_add(xe_mul(m[0], Y), xe_neg(xe_mul(m[2], X))), det), px);
+            if (xe_lt1(rx) && xe_lt1(ry) && std::hypot(xe_double(rx), xe_double(ry)) < 1.) { *order = i; break; }
+        }
+    }
+    return 0;
+}
+
+int fsg_newton_eval(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fsg_mp_values *out) {
+    int rc = check_desc(d, 8.);
+    if (rc) return rc;
+    if (!out || order < 1 || out->cap < 0 || (out->cap > 0 && (!out->limbs || !out->neg || !out->exp)))
+        return set_error(FSG_E_INVALID, "bad Newton arguments");
+    const bool holo = d->kind != FSG_KIND_BS;
+    const int deg = d->kind == FSG_KIND_M2 ? 2 : d->degree;
+    std::unique_ptr<Team> team = make_team(d);
+    Arith A(d->n_limbs, d->frac_limbs, team.get());
+    Work w(A);
+    const Fx cre = load(A, d->c_re, d->c_re_neg), cim = load(A, d->c_im, d->c_im_neg);
+    int64_t n = 0;
+    const auto put = [&](const Fx &v, int64_t e) {
+        if (n < out->cap) {
+            std::memcpy(out->limbs + n * A.n, v.m.data(), sizeof(u64) * A.n);
+            out->neg[n] = v.neg ? 1 : 0;
+            out->exp[n] = e;
+        }
+        n++;
+    };
+    if (holo) {
+        Cx z, zn;
+        z.re = A.make(); z.im = A.make(); zn.re = A.make(); zn.im = A.make();
+        HoloDeriv D(A);
+        for (int64_t i = 1; i <= order; i++) {
+            D.step(A, z, deg, w);
+            cpow(A, zn, z, deg, w);
+            A.add(z.re, zn.re, cre);
+            A.add(z.im, zn.im, cim);
+            if (A.overflowed()) return set_error(FSG_E_RANGE, "orbit left the fixed-point range");
+            if (i == order || (divisors && order % i == 0)) {
+                put(z.re, 0); put(z.im, 0); put(D.dz.v[0], D.dz.e); put(D.dz.v[1], D.dz.e);
+            }
+        }
+    } else {
+        Fx x = A.make(), y = A.make();
+        BsDeriv D(A);
+        for (int64_t i = 1; i <= order; i++) {
+            D.step(A, d->flavor, x, y);
+            bs_step(A, d->flavor, x, y, cre, cim, w);
+            if (A.overflowed()) return set_error(FSG_E_RANGE, "orbit left the fixed-point range");
+        }
+        put(x, 0); put(y, 0);
+        for (int k = 0; k < 4; k++) put(D.D.v[k], D.D.e);
+    }
+    out->n = n;
+    if (n > out->cap) return set_error(FSG_E_INVALID, "value buffer too small");
     return 0;
 }
 

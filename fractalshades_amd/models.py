@@ -116,14 +116,14 @@ class Perturbation_mandelbrot(_HolomorphicBase):
                                calc_orbit=calc_orbit, backshift=backshift)
 
     def _ball_method(self, c, px, maxiter, M_divergence):
-        return fsFP.holo_ball_method(c, px, 2, maxiter, M_divergence, self.prec)
+        return _orbits().holo_ball_method(c, px, 2, maxiter, M_divergence, self.prec)
 
     def find_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
-        return fsFP.find_nucleus_m2(c, order, eps_pixel, prec or self.prec,
+        return _orbits().find_nucleus_m2(c, order, eps_pixel, prec or self.prec,
                                     max_newton or 80, eps_cv)
 
     def find_any_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
-        return fsFP.find_any_nucleus(c, order, 2, eps_pixel, prec or self.prec,
+        return _orbits().find_any_nucleus(c, order, 2, eps_pixel, prec or self.prec,
                                      max_newton or 80, eps_cv)
 
     def _nucleus_size_estimate(self, c0, order):
@@ -168,14 +168,14 @@ class Perturbation_mandelbrot_N(_HolomorphicBase):
                                calc_orbit=calc_orbit, backshift=backshift)
 
     def _ball_method(self, c, px, maxiter, M_divergence):
-        return fsFP.holo_ball_method(c, px, self.exponent, maxiter, M_divergence, self.prec)
+        return _orbits().holo_ball_method(c, px, self.exponent, maxiter, M_divergence, self.prec)
 
     def find_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
         raise NotImplementedError(
             "Divide by undesired roots technique not implemented, Use 'find_any_nucleus'")
 
     def find_any_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
-        return fsFP.find_any_nucleus(c, order, self.exponent, eps_pixel, prec or self.prec,
+        return _orbits().find_any_nucleus(c, order, self.exponent, eps_pixel, prec or self.prec,
                                      max_newton or 80, eps_cv)
 
     def _nucleus_size_estimate(self, c0, order):
@@ -250,11 +250,11 @@ class Perturbation_burning_ship(PerturbationFractal):
         return {"set_state": set_state, "initialize": initialize, "iterate": iterate}
 
     def _ball_method(self, c, px, maxiter, M_divergence):
-        return fsFP.bs_ball_method(c, px, self.flavor_int, maxiter, M_divergence, self.prec)
+        return _orbits().bs_ball_method(c, px, self.flavor_int, maxiter, M_divergence, self.prec)
 
     def find_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
         raise NotImplementedError("use find_any_nucleus")
 
     def find_any_nucleus(self, c, order, eps_pixel, prec=None, max_newton=None, eps_cv=None):
-        return fsFP.bs_find_any_nucleus(c, order, self.flavor_int, eps_pixel,
+        return _orbits().bs_find_any_nucleus(c, order, self.flavor_int, eps_pixel,
                                         prec or self.prec, max_newton or 80, eps_cv)

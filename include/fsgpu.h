@@ -315,6 +315,45 @@ typedef struct fsg_orbit_out {
 
 int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o);
 
+/*
+ * Period of the reference point by the ball method: the first i <= max_iter
+ * with |z_i / (dz_i/dc)| < px (burning ship: the 2x2 solve of (x_i, y_i)
+ * against the Jacobian in (a, b), both components and their hypot below px),
+ * or -1 when |z_i| > M first or no i qualifies.  Replaces
+ *   perturbation_mandelbrotN_select_ball_method (FP_loop.pyx:631-758)
+ *   perturbation_nonholomorphic_ball_method     (FP_loop.pyx:2357-2462).
+ * px = px_mant * 2^px_exp.  The orbit is iterated in fixed point as in
+ * fsg_ref_orbit; the derivatives in block floating point (fixed-point
+ * mantissas sharing a binary exponent); the ratio test in binary64 with a
+ * wide exponent (the reference tests it at 54 bits).  d->M is the escape
+ * radius M_divergence; d->max_iter the iteration limit.
+ */
+int fsg_ball_period(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int64_t *order);
+
+/* Multi-precision values handed back to the caller:
+ * value[i] = (neg[i] ? -1 : 1) * limbs[i] * 2^(exp[i] - 64 * frac_limbs),
+ * limbs[i] the n_limbs little-endian words of the i-th magnitude. */
+typedef struct fsg_mp_values {
+    int64_t cap;           /* capacity, in values */
+    uint64_t *limbs;       /* [cap][n_limbs] */
+    int32_t *neg;          /* [cap] */
+    int64_t *exp;          /* [cap] */
+    int64_t n;             /* out: values written */
+} fsg_mp_values;
+
+/*
+ * The orbit and its derivative at c after `order` steps, for one Newton step
+ * of the nucleus search; the caller forms the step at its own precision.
+ * Holomorphic: (Re z, Im z, Re dz/dc, Im dz/dc) after step `order`, and,
+ * with `divisors`, first after every step i that divides order (the
+ * divisor-removal Newton of perturbation_mandelbrot_find_nucleus,
+ * FP_loop.pyx:900-1116); burning ship: (x, y, dx/da, dx/db, dy/da, dy/db).
+ * Replaces the inner loops of those Newton searches
+ * (FP_loop.pyx:900-1116, 1159-1329, 2564-2752).  FSG_E_RANGE when the orbit
+ * leaves the fixed-point range.
+ */
+int fsg_newton_eval(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fsg_mp_values *out);
+
 #ifdef __cplusplus
 }
 #endif

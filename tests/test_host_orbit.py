@@ -228,3 +228,83 @@ def test_ref_path_kat_full():
     and the membership checks of numba_path_loop (test_FP_loop.py:66-80)."""
     max_iter = KATS["ref_path"]["max_iter"]
     _check_ref_path(_ref_path_frame(max_iter), max_iter)
+
+
+# ---- period and nucleus ------------------------------------------------------
+def test_ball_method_kat_native():
+    """tests/test_FP_loop.py:166-193 with the fixed-point loop: period 4252."""
+    k = KATS["ball_method"]
+    prec = k["precision_bits"]
+    order = fo.holo_ball_method(_c(k["x"], k["y"], prec), mpmath.mpf(k["radius"]), 2,
+                                k["maxiter"], k["M_divergence"], prec)
+    assert order == k["order"]
+
+
+@pytest.mark.parametrize("degree", [2, 3, 5])
+@pytest.mark.parametrize("seed", range(4))
+def test_holo_ball_method_equals_mpmath(degree, seed):
+    rng = np.random.default_rng(100 * degree + seed)
+    prec = 160
+    c = _c(repr(rng.uniform(-1.6, 0.4)), repr(rng.uniform(0., 1.0)), prec)
+    for px in ("1e-3", "1e-9", "1e-25"):
+        a = fo.holo_ball_method(c, mpmath.mpf(px), degree, 4000, 1000., prec)
+        b = ml.holo_ball_method(c, mpmath.mpf(px), degree, 4000, 1000., prec)
+        assert a == b
+
+
+@pytest.mark.parametrize("flavor", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("seed", range(3))
+def test_bs_ball_method_equals_mpmath(flavor, seed):
+    rng = np.random.default_rng(10 * flavor + seed)
+    prec = 160
+    c = _c(repr(rng.uniform(-1.9, 0.4)), repr(rng.uniform(-1.0, 0.2)), prec)
+    for px in ("1e-3", "1e-9", "1e-25"):
+        a = fo.bs_ball_method(c, mpmath.mpf(px), flavor, 3000, 1000., prec)
+        b = ml.bs_ball_method(c, mpmath.mpf(px), flavor, 3000, 1000., prec)
+        assert a == b
+
+
+def test_newton_values_equal_mpmath():
+    """fsg_newton_eval's orbit point and derivative at the order and at its
+    divisors agree with a 200-bit-deeper mpmath orbit to the working
+    precision (the fixed-point loop carries 128 guard bits)."""
+    from fractalshades_amd import _native
+    prec = 300
+    c = _c("-1.7548776662466927600495", "0.0003", prec)
+    order = 12
+    vals = fo._newton_values(_native.FSG_KIND_M2, c, 2, 1, order, prec, True)
+    with mpmath.workprec(prec + 200):
+        z, dz, want = mpmath.mpc(0), mpmath.mpc(0), []
+        for i in range(1, order + 1):
+            dz = 2 * dz * z + 1
+            z = z * z + c
+            if order % i == 0:
+                want += [z.real, z.imag, dz.real, dz.imag]
+        assert len(vals) == len(want) == 4 * 6
+        for v, w in zip(vals, want):
+            assert abs(v - w) <= max(abs(w), 1) * mpmath.ldexp(1, -prec)
+
+
+@pytest.mark.parametrize("name", ["M2_divref5", "M2_int32", "BS_ex14", "mn3", "mn5"])
+def test_frame_reference_equals_mpmath(name):
+    """The whole reference-point search of a test frame (ball-method period,
+    Newton nucleus, orbit) with either backend: same period, nucleus within
+    the working precision, identical complex128 orbit and Xrange register."""
+    import frames
+    out = {}
+    old = settings.orbit_backend
+    try:
+        for be in ("mpmath", "native"):
+            settings.orbit_backend = be
+            f = frames.mn_frame(int(name[2:])) if name.startswith("mn") else frames.make(name)
+            f.get_FP_orbit()
+            out[be] = (f.FP_params, f.Zn_path, f.prec)
+    finally:
+        settings.orbit_backend = old
+    (pm, Zm, prec), (pn, Zn, _) = out["mpmath"], out["native"]
+    assert pm["order"] == pn["order"] is not None
+    assert pm["div_iter"] == pn["div_iter"]
+    with mpmath.workprec(prec):
+        assert abs(pm["ref_point"] - pn["ref_point"]) <= abs(pm["ref_point"]) * mpmath.ldexp(1, -prec + 4)
+    np.testing.assert_array_equal(Zm.view(np.uint64), Zn.view(np.uint64))
+    assert pm["xr"] == pn["xr"]
