@@ -86,6 +86,7 @@ class PostDesc(ctypes.Structure):
         ("holomorphic", ctypes.c_int32), ("dzndc_row", ctypes.c_int32),
         ("d", ctypes.c_double), ("a_d", ctypes.c_double), ("M", ctypes.c_double),
         ("floor_iter", ctypes.c_double), ("px_snap", ctypes.c_double),
+        ("skew", ctypes.c_double * 4),
     ]
 
 
@@ -176,8 +177,9 @@ def load(path=None):
             "fsg_server_submit": ([vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
                                   ctypes.c_int),
             "fsg_server_wait": ([vp, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)], ctypes.c_int),
-            "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp], ctypes.c_int),
-            "fsg_server_submit_layers": ([vp, vp, vp, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp,
+            "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp, vp],
+                                    ctypes.c_int),
+            "fsg_server_submit_layers": ([vp, vp, vp, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp, vp,
                                           ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "fsg_host_alloc": ([ctypes.c_size_t], vp),
             "fsg_host_free": ([vp], ctypes.c_int),

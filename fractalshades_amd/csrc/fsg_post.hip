@@ -10,9 +10,16 @@
 //   burning ship |dz_n/dc| is the larger singular value of the Jacobian,
 //   (hypot(a + d, b - c) + hypot(a - d, b + c)) / 2, and values below
 //   px_snap are snapped to 0.
+//   Attr_normal_pp with kind "potential" / "infinity" (postproc.py:572-628):
+//     holomorphic z_n / (dz_n/dc); burning ship J^T z_n with J the Jacobian
+//     d(x, y)/d(a, b); then the transposed skew (apply_unskew_1d,
+//     core.py:3147-3158) and division by the modulus.  The direction of
+//     z_n / (dz_n/dc) is that of z_n conj(dz_n/dc), which is formed instead
+//     (after a power-of-two scaling of the derivative, so that no product
+//     overflows): the unit vector is the same.
 // Computed in binary64, stored as binary32 (the reference's default
 // post-processing dtype, settings.py:82).  Pixels that did not diverge get
-// NaN in both layers: the reference's formula has no meaning there and its
+// NaN in every layer: the reference's formulas have no meaning there and its
 // layers mask them by stop reason.
 #include "fsgpu.h"
 #include "fsg_internal.hpp"
@@ -33,9 +40,10 @@ struct PostArgs {
     int64_t n, ld;
     const int32_t *si;
     const int8_t *sr;
-    float *nu, *dem;
+    float *nu, *dem, *nrm;
     int32_t holo, row_dc;
     double k, log_mk, log_d, floor_iter, px_snap;
+    double s00, s01, s10, s11;
 };
 
 __global__ void __launch_bounds__(256) k_postproc(const PostArgs p) {
@@ -44,6 +52,7 @@ __global__ void __launch_bounds__(256) k_postproc(const PostArgs p) {
         if (p.sr[i] != 1) {
             if (p.nu) p.nu[i] = __builtin_nanf("");
             if (p.dem) p.dem[i] = __builtin_nanf("");
+            if (p.nrm) { p.nrm[2 * i] = __builtin_nanf(""); p.nrm[2 * i + 1] = __builtin_nanf(""); }
             continue;
         }
         double zr, zi;
@@ -74,6 +83,28 @@ __global__ void __launch_bounds__(256) k_postproc(const PostArgs p) {
             if (p.px_snap > 0. && v < p.px_snap) v = 0.;
             p.dem[i] = (float)v;
         }
+        if (p.nrm) {
+            double nx, ny;
+            if (p.holo) {
+                const int64_t r = (int64_t)p.row_dc * p.ld + i;
+                double dr = p.Z[2 * r], di = p.Z[2 * r + 1];
+                const int sh = -__builtin_amdgcn_frexp_exp(fmax(fabs(dr), fabs(di)));
+                dr = ldexp(dr, sh); di = ldexp(di, sh);
+                nx = zr * dr + zi * di;          // z conj(dz/dc)
+                ny = zi * dr - zr * di;
+            } else {
+                const double *J = p.Z + (int64_t)p.row_dc * p.ld + i;
+                double a = J[0], b = J[p.ld], c = J[2 * p.ld], d = J[3 * p.ld];
+                const int sh = -__builtin_amdgcn_frexp_exp(fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d))));
+                a = ldexp(a, sh); b = ldexp(b, sh); c = ldexp(c, sh); d = ldexp(d, sh);
+                nx = a * zr + c * zi;            // J^T z
+                ny = b * zr + d * zi;
+            }
+            const double ux = p.s00 * nx + p.s10 * ny, uy = p.s01 * nx + p.s11 * ny;
+            const double m = hypot(ux, uy);
+            p.nrm[2 * i] = (float)(ux / m);
+            p.nrm[2 * i + 1] = (float)(uy / m);
+        }
     }
 }
 
@@ -87,6 +118,9 @@ int fill_post(const fsg_post_desc *pp, PostArgs &a) {
     a.log_d = std::log(pp->d);
     a.floor_iter = pp->floor_iter;
     a.px_snap = pp->px_snap;
+    const bool skew = pp->skew[0] != 0. || pp->skew[1] != 0. || pp->skew[2] != 0. || pp->skew[3] != 0.;
+    a.s00 = skew ? pp->skew[0] : 1.; a.s01 = skew ? pp->skew[1] : 0.;
+    a.s10 = skew ? pp->skew[2] : 0.; a.s11 = skew ? pp->skew[3] : 1.;
     return 0;
 }
 
@@ -96,12 +130,13 @@ namespace fsg {
 
 // enqueue the layers of n pixels whose raw results sit in Z (row stride ld)
 int post_launch(const fsg_post_desc *pp, const double *Z, int64_t n, int64_t ld, const int32_t *si,
-                const int8_t *sr, float *nu, float *dem, hipStream_t stream) {
+                const int8_t *sr, float *nu, float *dem, float *nrm, hipStream_t stream) {
     PostArgs a;
     int rc = fill_post(pp, a);
     if (rc) return rc;
-    if (dem && pp->dzndc_row < 1) return set_error(FSG_E_INVALID, "DEM needs the dz/dc (Jacobian) rows");
-    a.Z = Z; a.n = n; a.ld = ld; a.si = si; a.sr = sr; a.nu = nu; a.dem = dem;
+    if ((dem || nrm) && pp->dzndc_row < 1)
+        return set_error(FSG_E_INVALID, "DEM and normal need the dz/dc (Jacobian) rows");
+    a.Z = Z; a.n = n; a.ld = ld; a.si = si; a.sr = sr; a.nu = nu; a.dem = dem; a.nrm = nrm;
     if (n <= 0) return 0;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
@@ -116,10 +151,11 @@ extern "C" {
 
 int fsg_postproc_device(const fsg_post_desc *pp, const double *d_Z, int64_t n, int64_t ld,
                         const int32_t *d_stop_iter, const int8_t *d_stop_reason, float *d_nu, float *d_dem,
-                        void *stream) {
+                        float *d_normal, void *stream) {
     if (n < 0 || ld < n || (n > 0 && (!d_Z || !d_stop_iter || !d_stop_reason)))
         return set_error(FSG_E_INVALID, "bad post-processing buffers");
-    return post_launch(pp, d_Z, n, ld, d_stop_iter, d_stop_reason, d_nu, d_dem, static_cast<hipStream_t>(stream));
+    return post_launch(pp, d_Z, n, ld, d_stop_iter, d_stop_reason, d_nu, d_dem, d_normal,
+                       static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

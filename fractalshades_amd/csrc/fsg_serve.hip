@@ -50,7 +50,7 @@ int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, doubl
                      int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks);
 size_t work_words();
 int post_launch(const fsg_post_desc *pp, const double *Z, int64_t n, int64_t ld, const int32_t *si,
-                const int8_t *sr, float *nu, float *dem, hipStream_t stream);
+                const int8_t *sr, float *nu, float *dem, float *nrm, hipStream_t stream);
 }  // namespace fsg
 
 using namespace fsg;
@@ -101,7 +101,7 @@ private:
     int64_t size_ = 0;
 };
 
-struct Layout { int64_t pix, z, u, si, sr, nu, dem, total; };
+struct Layout { int64_t pix, z, u, si, sr, nu, dem, nrm, total; };
 
 Layout layout_of(const fsg_ctx *c, int64_t n, bool layers = false) {
     Layout l;
@@ -112,7 +112,8 @@ Layout layout_of(const fsg_ctx *c, int64_t n, bool layers = false) {
     l.sr = l.si + al256(4 * n);
     l.nu = l.sr + al256(n);
     l.dem = l.nu + (layers ? al256(4 * n) : 0);
-    l.total = l.dem + (layers ? al256(4 * n) : 0);
+    l.nrm = l.dem + (layers ? al256(4 * n) : 0);
+    l.total = l.nrm + (layers ? al256(8 * n) : 0);
     return l;
 }
 
@@ -125,7 +126,7 @@ struct Req {
     double *Z_h = nullptr; int32_t *U_h = nullptr; int8_t *sr_h = nullptr; int32_t *si_h = nullptr;
     bool layers = false;                  // return nu / dem instead of Z / U
     fsg_post_desc pp{};
-    float *nu_h = nullptr, *dem_h = nullptr;
+    float *nu_h = nullptr, *dem_h = nullptr, *nrm_h = nullptr;
     int64_t n = 0;
     Layout lay{};
     int64_t off = -1;
@@ -253,12 +254,15 @@ void fsg_server::loop() {
                 if (ok && n > 0 && r->layers) {
                     float *nu = r->nu_h ? reinterpret_cast<float *>(r->dev + r->lay.nu) : nullptr;
                     float *dem = r->dem_h ? reinterpret_cast<float *>(r->dev + r->lay.dem) : nullptr;
+                    float *nrm = r->nrm_h ? reinterpret_cast<float *>(r->dev + r->lay.nrm) : nullptr;
                     const int prc = post_launch(&r->pp, reinterpret_cast<const double *>(r->dev + r->lay.z), n, n,
                                                 reinterpret_cast<const int32_t *>(r->dev + r->lay.si),
-                                                reinterpret_cast<const int8_t *>(r->dev + r->lay.sr), nu, dem, copy);
+                                                reinterpret_cast<const int8_t *>(r->dev + r->lay.sr), nu, dem, nrm,
+                                                copy);
                     if (prc) { fail_req(r, prc, fsg_last_error()); ok = false; }
                     if (ok && nu) ok = hipMemcpyAsync(r->nu_h, nu, (size_t)(4 * n), hipMemcpyDeviceToHost, copy) == hipSuccess;
                     if (ok && dem) ok = hipMemcpyAsync(r->dem_h, dem, (size_t)(4 * n), hipMemcpyDeviceToHost, copy) == hipSuccess;
+                    if (ok && nrm) ok = hipMemcpyAsync(r->nrm_h, nrm, (size_t)(8 * n), hipMemcpyDeviceToHost, copy) == hipSuccess;
                     ok = ok && hipMemcpyAsync(r->si_h, r->dev + r->lay.si, (size_t)(4 * n), hipMemcpyDeviceToHost, copy) == hipSuccess &&
                          hipMemcpyAsync(r->sr_h, r->dev + r->lay.sr, (size_t)n, hipMemcpyDeviceToHost, copy) == hipSuccess;
                     if (!ok) fail_req(r, FSG_E_HIP, "layer copy failed");
@@ -430,19 +434,19 @@ int fsg_server_submit(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t 
 }
 
 int fsg_server_submit_layers(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t n,
-                             const fsg_post_desc *pp, float *nu, float *dem, int8_t *stop_reason,
-                             int32_t *stop_iter, uint64_t *ticket) {
+                             const fsg_post_desc *pp, float *nu, float *dem, float *normal,
+                             int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket) {
     if (!s || !ctx || !ticket || !pp) return set_error(FSG_E_INVALID, "null argument");
     if (n < 0 || (n > 0 && (!c_pix || !stop_reason || !stop_iter)))
         return set_error(FSG_E_INVALID, "bad tile buffers");
-    if (!(pp->d > 1.) || !(pp->M > 0.) || (dem && pp->dzndc_row < 1) ||
+    if (!(pp->d > 1.) || !(pp->M > 0.) || ((dem || normal) && pp->dzndc_row < 1) ||
         pp->dzndc_row >= ctx_nz(ctx) || pp->holomorphic != ctx_holomorphic(ctx))
         return set_error(FSG_E_INVALID, "post-processing descriptor does not match the context");
     Req *r = new (std::nothrow) Req();
     if (!r) return set_error(FSG_E_NOMEM, "out of host memory");
     r->ctx = ctx; r->pix_h = c_pix; r->n = n;
     r->sr_h = stop_reason; r->si_h = stop_iter;
-    r->layers = true; r->pp = *pp; r->nu_h = nu; r->dem_h = dem;
+    r->layers = true; r->pp = *pp; r->nu_h = nu; r->dem_h = dem; r->nrm_h = normal;
     r->lay = layout_of(ctx, n, true);
     return submit_req(s, r, ticket);
 }

@@ -7,7 +7,11 @@ count (potential) and distance estimate (DEM).
   then the fractional part is folded into (-1, 0] and n adjusted;
 * DEM: DEM_pp (postproc.py:684-731): |z_n| ln|z_n| / |dz_n/dc|; for the
   burning ship |dz_n/dc| is the larger singular value of the Jacobian,
-  (hypot(a + d, b - c) + hypot(a - d, b + c)) / 2.
+  (hypot(a + d, b - c) + hypot(a - d, b + c)) / 2;
+* normal map: Attr_normal_pp kind "potential" with potential "infinity"
+  (postproc.py:572-628): z_n / (dz_n/dc), for the burning ship J^T z_n,
+  unskewed with the transposed skew (core.py:3147-3158), divided by its
+  modulus.
 
 Computed in float64 (the reference's default post-processing dtype is
 float32, settings.py:82; the parity checks need float64).  Also: the error
@@ -44,6 +48,20 @@ def dem(zn, dzndc=None, jac=None):
         abs_d = np.abs(np.asarray(dzndc))
     with np.errstate(divide="ignore", invalid="ignore"):
         return abs_zn * np.log(abs_zn) / abs_d
+
+
+def normal_map(zn, dzndc=None, jac=None, skew=None):
+    """Unit normal of the potential map, as a complex number."""
+    zn = np.asarray(zn, dtype=np.complex128)
+    if jac is not None:
+        dXdA, dXdB, dYdA, dYdB = (np.asarray(v, dtype=np.float64) for v in jac)
+        normal = (dXdA * zn.real + dYdA * zn.imag) + 1j * (dXdB * zn.real + dYdB * zn.imag)
+    else:
+        normal = zn / np.asarray(dzndc, dtype=np.complex128)
+    if skew is not None:
+        nx, ny = normal.real, normal.imag
+        normal = (skew[0, 0] * nx + skew[1, 0] * ny) + 1j * (skew[0, 1] * nx + skew[1, 1] * ny)
+    return normal / np.abs(normal)
 
 
 def fields_of(Z, codes, holomorphic=True):
@@ -102,8 +120,8 @@ def error_report(test, ref, holomorphic=True, d=2, M=1000.):
 def post_desc(f, calc_name, floor_iter=0., px_snap=None):
     """fsg_post_desc of a calculation: potential parameters of the model
     (potential_d, potential_a_d, potential_M, as Continuous_iter_pp reads
-    them, postproc.py:320-347) and the row of dz/dc (dX/da for the burning
-    ship) in its raw Z array."""
+    them, postproc.py:320-347), the row of dz/dc (dX/da for the burning
+    ship) in its raw Z array, and the frame's skew for the normal map."""
     from ._native import PostDesc
     state = f._calc_data[calc_name]["state"]
     names = list(state.codes[0])
@@ -113,4 +131,9 @@ def post_desc(f, calc_name, floor_iter=0., px_snap=None):
     d = float(getattr(f, "potential_d", 2))
     a_d = float(getattr(f, "potential_a_d", 1.))
     M = float(getattr(state, "potential_M", getattr(f, "potential_M_cutoff", 1000.)))
-    return PostDesc(int(holo), int(row), d, a_d, M, float(floor_iter), float(px_snap or 0.))
+    pp = PostDesc(int(holo), int(row), d, a_d, M, float(floor_iter), float(px_snap or 0.))
+    skew = getattr(f, "_skew", None)
+    sk = np.ravel(skew) if skew is not None else (1., 0., 0., 1.)
+    for i in range(4):
+        pp.skew[i] = float(sk[i])
+    return pp

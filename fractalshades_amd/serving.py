@@ -66,13 +66,17 @@ class PinnedTile:
 
 
 class LayerTile:
-    """c_pix in, the two post-processing layers (float32 nu, dem) with
-    stop_reason and stop_iter out, in pinned memory."""
+    """c_pix in, the post-processing layers (float32 nu, dem, and with
+    normal=True the unit normal as float32 [n, 2]) with stop_reason and
+    stop_iter out, in pinned memory."""
 
-    def __init__(self, n):
+    def __init__(self, n, normal=False):
         lib = load()
         specs = [("c_pix", (n,), np.complex128), ("nu", (n,), np.float32), ("dem", (n,), np.float32),
                  ("stop_reason", (1, n), np.int8), ("stop_iter", (1, n), np.int32)]
+        if normal:
+            specs.append(("normal", (n, 2), np.float32))
+        self.normal = None
         self._ptrs = []
         for name, shape, dt in specs:
             nbytes = int(np.prod(shape)) * np.dtype(dt).itemsize
@@ -127,12 +131,14 @@ class TileServer:
         return t.value
 
     def submit_layers(self, ctx, tile, post):
-        """Queue a request returning the nu / dem layers (LayerTile) instead
-        of the raw fields; post is a PostDesc (postproc.post_desc)."""
+        """Queue a request returning the nu / dem (/ normal) layers
+        (LayerTile) instead of the raw fields; post is a PostDesc
+        (postproc.post_desc)."""
         t = ctypes.c_uint64()
+        nrm = tile.normal.ctypes.data if tile.normal is not None else None
         check(load().fsg_server_submit_layers(
             self._h, ctx.handle, tile.c_pix.ctypes.data, tile.c_pix.shape[0], ctypes.byref(post),
-            tile.nu.ctypes.data, tile.dem.ctypes.data, tile.stop_reason.ctypes.data,
+            tile.nu.ctypes.data, tile.dem.ctypes.data, nrm, tile.stop_reason.ctypes.data,
             tile.stop_iter.ctypes.data, ctypes.byref(t)))
         with self._lock:
             self._keep[t.value] = (ctx, tile, post)

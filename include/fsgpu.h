@@ -208,13 +208,17 @@ typedef struct fsg_post_desc {
     double M;                /* potential_M (the escape radius) */
     double floor_iter;       /* Continuous_iter_pp floor_iter */
     double px_snap;          /* DEM_pp px_snap; <= 0 for none */
+    double skew[4];          /* the frame's skew (s00, s01, s10, s11); all zero = none */
 } fsg_post_desc;
 
 /* Enqueue the layers of n pixels (Z rows of stride ld >= n) on stream;
- * nu or dem may be null to skip that layer. */
+ * nu, dem or normal may be null to skip that layer.  normal: float32 [n][2],
+ * the unit normal of the potential map (Attr_normal_pp kind "potential",
+ * postproc.py:572-628: z_n / (dz_n/dc), for the burning ship J^T z_n; then
+ * unskewed with the transposed skew, core.py:3147-3158, and normalised). */
 int fsg_postproc_device(const fsg_post_desc *pp, const double *d_Z, int64_t n, int64_t ld,
                         const int32_t *d_stop_iter, const int8_t *d_stop_reason, float *d_nu,
-                        float *d_dem, void *stream);
+                        float *d_dem, float *d_normal, void *stream);
 
 /*
  * ---- Serving loop ----------------------------------------------------------
@@ -258,11 +262,11 @@ int fsg_server_clock(fsg_server *srv, double *ms);
 int fsg_server_submit(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z,
                       int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
 /* The same request returning the post-processing layers (nu, dem: float32
- * [n], either may be null) with stop_reason and stop_iter instead of the raw
- * fields: the raw results stay on the device. */
+ * [n]; normal: float32 [n][2]; any may be null) with stop_reason and
+ * stop_iter instead of the raw fields: the raw results stay on the device. */
 int fsg_server_submit_layers(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n,
-                             const fsg_post_desc *pp, float *nu, float *dem, int8_t *stop_reason,
-                             int32_t *stop_iter, uint64_t *ticket);
+                             const fsg_post_desc *pp, float *nu, float *dem, float *normal,
+                             int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
 /* Wait for a request (timeout_ms < 0: no limit).  Returns its status, or
  * FSG_PENDING on timeout; after a return other than FSG_PENDING the ticket
  * is released. */

@@ -143,12 +143,13 @@ def test_shared_context_rejects_other_orbit():
 
 @pytest.mark.parametrize("name,model_kw", [("M2_E20", None), ("BS_ex14", {"flavor": "Burning ship"})])
 def test_server_layers_match_host_postproc(name, model_kw):
-    """fsg_server_submit_layers: the continuous iteration count and the DEM
-    computed on the device from the raw results equal the host formulas of
-    postproc.py (Continuous_iter_pp / DEM_pp restated in numpy) on the raw
+    """fsg_server_submit_layers: the continuous iteration count, the DEM and
+    the normal map computed on the device from the raw results equal the host
+    formulas of postproc.py (Continuous_iter_pp / DEM_pp / Attr_normal_pp
+    restated in numpy, the burning-ship frame with its skew) on the raw
     results of fsg_tile, to float32 resolution; NaN where the pixel did not
     diverge."""
-    from fractalshades_amd.postproc import continuous_iter, dem, post_desc
+    from fractalshades_amd.postproc import continuous_iter, dem, normal_map, post_desc
     from fractalshades_amd.serving import LayerTile
     kw = {"max_iter": 6000} if name.startswith("BS") else {}
     f = frames.make(name, nx=64, model_kw=model_kw, **kw)
@@ -161,8 +162,11 @@ def test_server_layers_match_host_postproc(name, model_kw):
     d = f.potential_d
     nu_ref = continuous_iter(zn[esc], si[0][esc], d=d, a_d=f.potential_a_d, M=1e3)
     dem_ref = dem(zn[esc], dzndc=Z[-1][esc]) if holo else dem(zn[esc], jac=Z[2:6, esc])
+    nrm_ref = (normal_map(zn[esc], dzndc=Z[-1][esc], skew=f._skew) if holo
+               else normal_map(zn[esc], jac=Z[2:6, esc], skew=f._skew))
+    assert holo or f._skew is not None
     pp = post_desc(f, "t")
-    t = LayerTile(c.size)
+    t = LayerTile(c.size, normal=True)
     t.c_pix[:] = c
     srv = TileServer(lanes=2, chunk_px=1024)
     try:
@@ -175,4 +179,7 @@ def test_server_layers_match_host_postproc(name, model_kw):
     assert np.all(np.isnan(t.nu[~esc])) and np.all(np.isnan(t.dem[~esc]))
     np.testing.assert_allclose(t.nu[esc], nu_ref.astype(np.float32), rtol=2e-7, atol=0)
     np.testing.assert_allclose(t.dem[esc], dem_ref.astype(np.float32), rtol=2e-7, atol=0)
+    assert np.all(np.isnan(t.normal[~esc]))
+    np.testing.assert_allclose(t.normal[esc, 0], nrm_ref.real.astype(np.float32), rtol=0, atol=1.2e-7)
+    np.testing.assert_allclose(t.normal[esc, 1], nrm_ref.imag.astype(np.float32), rtol=0, atol=1.2e-7)
     t.close()
