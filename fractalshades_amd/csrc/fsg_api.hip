@@ -496,8 +496,11 @@ int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
         if (be != hipSuccess) return cleanup(fail(FSG_E_HIP, std::string("BLA build: ") + hipGetErrorString(be)));
         if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(FSG_E_HIP, "BLA build sync"));
     }
-    if (c->d.precision == FSG_PREC_FP32X) {
-        if (!f32_supported(cfg_of(c))) return cleanup(fail(FSG_E_UNSUPPORTED, "no FP32X kernel for this model"));
+    if (c->d.precision == FSG_PREC_FP32X && !f32_supported(cfg_of(c)))
+        return cleanup(fail(FSG_E_UNSUPPORTED, "no FP32X kernel for this model"));
+    // binary32 copies of the holomorphic tables (the burning-ship form
+    // converts its real tables as it reads them)
+    if (c->d.precision == FSG_PREC_FP32X && c->d.kind != FSG_KIND_BS) {
         hipError_t xe;
         void *p;
         const size_t L1 = (size_t)L + 1;

@@ -260,11 +260,166 @@ struct HoloF32 {
     }
 };
 
+// ---------------------------------------------------------------------------
+// burning-ship family: real quantities as a binary32 mantissa in [1, 2)
+// (or 0) and an int32 exponent; the step and Jacobian formulas are the
+// binary64 kernels' own templates (models.hpp bs_zn / bs_hess,
+// burning_ship.py:535-859) instantiated on this type
+struct xsr { float m; int32_t e; };
+
+__device__ __forceinline__ xsr xsr_n(float m, int32_t e) {
+    const int32_t k = fexp(m) - 1;
+    xsr r;
+    r.m = ldexpf(m, -k);
+    r.e = (m == 0.f) ? XS_ZERO_E : e + k;
+    return r;
+}
+__device__ __forceinline__ xsr xsr_from(double m, int32_t e) {
+    const int32_t k = __builtin_amdgcn_frexp_exp(m) - 1;
+    return xsr_n((float)ldexp(m, -k), (m == 0.) ? XS_ZERO_E : e + k);
+}
+__device__ __forceinline__ double xsr_to_d(xsr a) { return ldexp((double)a.m, max(a.e, -2000)); }
+__device__ __forceinline__ xsr operator+(xsr a, xsr b) {
+    const int32_t E = max(a.e, b.e);
+    return xsr_n(ldexpf(a.m, max(a.e - E, -200)) + ldexpf(b.m, max(b.e - E, -200)), E);
+}
+__device__ __forceinline__ xsr operator-(xsr a) { a.m = -a.m; return a; }
+__device__ __forceinline__ xsr operator-(xsr a, xsr b) { return a + (-b); }
+__device__ __forceinline__ xsr operator*(xsr a, xsr b) { return xsr_n(a.m * b.m, a.e + b.e); }
+__device__ __forceinline__ xsr operator*(double s, xsr a) { return xsr_n((float)s * a.m, a.e); }
+__device__ __forceinline__ xsr mul_s(double s, xsr a) { return xsr_n((float)s * a.m, a.e); }
+__device__ __forceinline__ bool xsr_le(xsr a, xsr b) {   // a <= b
+    const int32_t E = max(a.e, b.e);
+    return ldexpf(a.m, max(a.e - E, -200)) <= ldexpf(b.m, max(b.e - E, -200));
+}
+__device__ __forceinline__ xsr absv(xsr x) { x.m = fabsf(x.m); return x; }
+__device__ __forceinline__ double sgn(xsr x) { return x.m < 0.f ? -1. : 1.; }
+// |X + x| - |X| and its partial derivatives (burning_ship.py:19-60)
+__device__ __forceinline__ xsr diffabs(xsr X, xsr x) {
+    if (X.m >= 0.f) { if ((X + x).m >= 0.f) return x; return -(2. * X + x); }
+    if ((X + x).m <= 0.f) return -x;
+    return 2. * X + x;
+}
+__device__ __forceinline__ double ddX(xsr X, xsr x) {
+    if (X.m >= 0.f) return (X + x).m >= 0.f ? 0. : -2.;
+    return (X + x).m <= 0.f ? 0. : 2.;
+}
+__device__ __forceinline__ double ddx(xsr X, xsr x) {
+    if (X.m >= 0.f) return (X + x).m >= 0.f ? 1. : -1.;
+    return (X + x).m <= 0.f ? -1. : 1.;
+}
+
+template <int FL>
+struct BsF32 {
+    struct State { xsr x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; };
+
+    __device__ static xsr PATH(const Tables &t, int q, int32_t i) { return xsr_from(t.bs[q][i], t.bs_e[q][i]); }
+    __device__ static void ZR(const Tables &t, int32_t i, xsr &X, xsr &Y) {
+        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
+        if (j >= 0) { X = xsr_from(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xsr_from(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
+        else { const cplx R = t.Zn[i]; X = xsr_from(R.re, 0); Y = xsr_from(R.im, 0); }
+    }
+
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
+        s.a = xsr_from(a.dx_m * (a.lin[0] * px + a.lin[1] * py), a.dx_e) + xsr_from(a.drift_m[0], a.drift_e[0]);
+        s.b = xsr_from(a.dx_m * (a.lin[2] * px + a.lin[3] * py), a.dx_e) + xsr_from(a.drift_m[1], a.drift_e[1]);
+        const xsr z0 = xsr{0.f, XS_ZERO_E};
+        s.x = z0; s.y = z0; s.P[0] = s.P[1] = s.P[2] = s.P[3] = z0;
+        s.xs = 0.; s.ys = 0.;
+        s.w = 0; s.n = 0; s.dyn = true;
+    }
+
+    __device__ static int step(State &s, const KArgs &a, Counters &k) {
+        const Tables &t = a.t;
+        while (t.bla_on && (s.w & 7) == 0) {
+            int32_t idx;
+            const int32_t st = bla_find(t, hypot2(s.xs, s.ys), s.w, idx);
+            k.tries++;
+            if (st == 0) break;
+            {
+                const double *M = t.M + 8 * idx;
+                s.n += st; s.w += st;
+                const xsr nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
+                const xsr ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
+                s.x = nx; s.y = ny;
+                s.xs = xsr_to_d(s.x); s.ys = xsr_to_d(s.y);
+                if (a.calc_dc) {
+                    const xsr p0 = M[0] * s.P[0] + M[1] * s.P[2];
+                    const xsr p1 = M[0] * s.P[1] + M[1] * s.P[3];
+                    const xsr p2 = M[2] * s.P[0] + M[3] * s.P[2];
+                    const xsr p3 = M[2] * s.P[1] + M[3] * s.P[3];
+                    s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
+                }
+                k.bla++; k.bla_it += st;
+            }
+        }
+        s.n += 1; k.plain++;
+        xsr X, Y;
+        ZR(t, s.w, X, Y);
+        if (a.calc_dc) {
+            const xsr z0 = xsr{0.f, XS_ZERO_E};
+            xsr Rd[4] = {z0, z0, z0, z0};
+            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = PATH(t, q, s.w);
+            bs_hess<FL>(s.x, s.y, s.P, X, Y, Rd);
+        }
+        bs_zn<FL>(s.x, s.y, X, Y, s.a, s.b);
+        s.xs = xsr_to_d(s.x); s.ys = xsr_to_d(s.y);
+        if (s.n >= a.max_iter) return 0;
+        s.w += 1;
+        if (s.w >= t.order) s.w = 0;
+        // escape and rebase tests in binary64, on the binary64 orbit
+        const cplx R1 = t.Zn[s.w];
+        const double XX = s.xs + R1.re, YY = s.ys + R1.im;
+        if (XX * XX + YY * YY > a.M2) return 1;
+        if (s.w >= t.div_iter - 1) {
+            s.xs = XX; s.ys = YY;
+            s.x = xsr_from(XX, 0); s.y = xsr_from(YY, 0);
+            if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
+            s.w = 0;
+            k.reb++;
+            return -1;
+        }
+        s.dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
+        if (s.dyn) {
+            xsr X1, Y1;
+            ZR(t, s.w, X1, Y1);
+            const xsr XXx = s.x + X1, YYx = s.y + Y1;
+            if (xsr_le(XXx * XXx + YYx * YYx, s.x * s.x + s.y * s.y)) {
+                s.x = XXx; s.y = YYx;
+                s.xs = xsr_to_d(XXx); s.ys = xsr_to_d(YYx);
+                if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
+                s.w = 0;
+                k.reb++;
+            }
+        }
+        return -1;
+    }
+
+    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
+        const int64_t n = a.ld;
+        const cplx R = a.t.Zn[s.w];
+        a.Z[i] = s.xs + R.re; a.Z[n + i] = s.ys + R.im;
+        if (a.calc_dc)
+            for (int q = 0; q < 4; q++) a.Z[(2 + q) * n + i] = xsr_to_d(s.P[q] + PATH(a.t, q, s.w));
+        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
+    }
+};
+
 typedef void (*tile_fn)(const KArgs);
 template <class Model> static tile_fn fn_of() { return k_tile<Model>; }
 
 static tile_fn select_f32(const LaunchCfg &c) {
     if (c.kind == KIND_M2) return fn_of<HoloF32<KIND_M2, 2>>();
+    if (c.kind == KIND_BS) {
+        switch (c.flavor) {
+            case 1: return fn_of<BsF32<1>>();
+            case 2: return fn_of<BsF32<2>>();
+            case 3: return fn_of<BsF32<3>>();
+            case 4: return fn_of<BsF32<4>>();
+            case 5: return fn_of<BsF32<5>>();
+            default: return nullptr;
+        }
+    }
     if (c.kind == KIND_MN) {
         switch (c.degree) {
 #define FSG_MN(D) case D: return fn_of<HoloF32<KIND_MN, D>>();

@@ -36,7 +36,8 @@ def _mix():
     out += [("Mn3", a, c)]
     f = frames.make("BS_ex14", nx=56, max_iter=4000)
     a, c = frames.frame_inputs(f)
-    out += [("BS_ex14", a, c), ("BS_ex14 xr", with_fields(a, xr_mode=1), c)]
+    out += [("BS_ex14", a, c), ("BS_ex14 xr", with_fields(a, xr_mode=1), c),
+            ("BS_ex14 fp32x", with_fields(a, precision=1), c)]
     return out
 
 
