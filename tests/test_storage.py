@@ -1,0 +1,92 @@
+"""On-disk raw results in the reference's layout (fractalshades_amd.storage,
+src/fractalshades/core.py:2125-2512): file names, memmap shapes and dtypes,
+the tile report, the fingerprint test; on the GPU, stored tiles equal the
+tiles computed one by one, and an interrupted calculation resumes with only
+its missing tiles."""
+import os
+
+import numpy as np
+import pytest
+
+import frames
+from fractalshades_amd import settings, storage
+
+
+def _frame(tmp_path, name="M2_E20", nx=250, **kw):
+    f = frames.make(name, nx=nx, **kw)
+    f.directory = str(tmp_path)
+    return f
+
+
+def test_layout_and_fingerprint(tmp_path):
+    f = _frame(tmp_path)
+    assert f.chunks_count == 4 and settings.chunk_size == 200      # 200 + 50 per axis
+    storage.init_files(f, "t")
+    paths = storage.data_path(f, "t")
+    n_pts = f.nx * f.ny
+    state = f._calc_data["t"]["state"]
+    n_Z, n_U = len(state.codes[0]), len(state.codes[1])
+    for key, shape, dt in (("Z", (n_Z, n_pts), np.complex128), ("U", (n_U, n_pts), np.int32),
+                           ("stop_reason", (1, n_pts), np.int8), ("stop_iter", (1, n_pts), np.int32)):
+        assert paths[key].endswith(f"data{os.sep}t_{key}.arr")
+        a = np.load(paths[key], mmap_mode="r")
+        assert a.shape == shape and a.dtype == dt
+    rep = np.load(storage.report_path(f, "t"))
+    assert rep.dtype == np.int32 and rep.shape == (4, 4)
+    pts = [(ixx - ix) * (iyy - iy) for ix, ixx, iy, iyy in f.chunk_slices()]
+    assert list(rep[:, 2]) == pts and rep[0, 0] == 0 and list(rep[1:, 0]) == list(rep[:-1, 1])
+    assert rep[-1, 1] == n_pts and not rep[:, 3].any()
+    assert storage.res_available(f, "t")
+    assert not storage.res_available(f, "t", next(iter(f.chunk_slices())))
+    fp = storage.fingerprint(f, "t")
+    assert fp["calc_class"] == "Perturbation_mandelbrot" and fp["calc_callable"] == "calc_std_div"
+    assert fp["calc_kwargs"]["max_iter"] == 5000
+    # another max_iter: the stored results no longer match
+    g = _frame(tmp_path, max_iter=4000)
+    assert not storage.res_available(g, "t")
+
+
+def test_subset_not_supported(tmp_path):
+    f = _frame(tmp_path, nx=16)
+    f._calc_data["t"]["state"].subset = np.ones(f.nx * f.ny, dtype=bool)
+    with pytest.raises(NotImplementedError):
+        storage.init_files(f, "t")
+
+
+def _tiles_equal_one_by_one(f):
+    for cs in f.chunk_slices():
+        _, c_pix, Z, U, sr, si = storage.reload_data(f, cs, "t")
+        ret, (c1, Z1, U1, sr1, si1) = f.compute_tile("t", cs)
+        assert ret == 0
+        np.testing.assert_array_equal(c_pix, c1)
+        for a, b in ((Z, Z1), (U, U1), (sr, sr1), (si, si1)):
+            np.testing.assert_array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["M2_E20", "BS_ex14"])
+def test_stored_tiles_equal_tiles_one_by_one(tmp_path, name):
+    f = _frame(tmp_path, name, nx=250, **({"max_iter": 4000} if name.startswith("BS") else {}))
+    print(f"case storage {name}", flush=True)
+    assert storage.calc_to_disk(f, "t") == 4
+    assert all(storage.res_available(f, "t", cs) for cs in f.chunk_slices())
+    _tiles_equal_one_by_one(f)
+    assert storage.calc_to_disk(f, "t") == 0          # nothing left to do
+
+
+@pytest.mark.gpu
+def test_resume_computes_only_missing_tiles(tmp_path):
+    f = _frame(tmp_path)
+    print("case storage resume", flush=True)
+    storage.calc_to_disk(f, "t")
+    # forget two tiles, as after an interruption
+    rep = np.lib.format.open_memmap(storage.report_path(f, "t"), mode="r+")
+    rep[1, 3] = 0
+    rep[3, 3] = 0
+    beg, end = int(rep[1, 0]), int(rep[1, 1])
+    del rep
+    z = np.lib.format.open_memmap(storage.data_path(f, "t")["Z"], mode="r+")
+    z[:, beg:end] = 0
+    del z
+    assert storage.calc_to_disk(f, "t") == 2
+    _tiles_equal_one_by_one(f)
