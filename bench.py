@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--deep", default="C2_2000", help="second frame (1e-2000); '' to skip")
+    ap.add_argument("--extra", default="C3_d3,C3_d4,C3_d6",
+                    help="further workloads reported under config.frames (fewer steps); '' to skip")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
     ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
@@ -261,19 +263,28 @@ def main():
     dev = torch.device("cuda:0")
     frames = {}
     main_leg = None
-    for wname in [args.workload] + ([args.deep] if args.deep else []):
+    def cached(name):
+        try:
+            return configs.center_for(name, compute=False) is not None
+        except KeyError:
+            return False
+    extra = [x for x in args.extra.split(",") if x and cached(x)]
+    for wname in [args.workload] + ([args.deep] if args.deep else []) + extra:
         w = configs.WORKLOADS[wname]
+        steps, warmup = (args.steps, args.warmup) if wname not in extra else \
+            (max(2, args.steps // 5), min(args.warmup, 1))
         t0 = time.perf_counter()
         f = configs.build(w)
         cia = f._calc_data["bench"]["cycle_indep_args"]
         prep_s = time.perf_counter() - t0
-        leg64 = Leg(f, cia, 0, torch, dev).run(args.steps, args.warmup)
-        entry = {"width": w.width, "pixels": leg64.n, "host_prep_s": round(prep_s, 2),
+        leg64 = Leg(f, cia, 0, torch, dev).run(steps, warmup)
+        entry = {"model": w.model + (f" d={w.degree}" if w.model == "MN" else ""), "steps": steps,
+                 "width": w.width, "pixels": leg64.n, "host_prep_s": round(prep_s, 2),
                  "ref_len": cia.fields["ref_len"], "ref_order": f.FP_params["order"],
                  "ref_div_iter": f.FP_params["div_iter"], "xr_mode": cia.fields["xr_mode"],
                  "fp64": leg64.summary()}
         if not args.no_fp32x:
-            leg32 = Leg(f, cia, 1, torch, dev).run(args.steps, args.warmup)
+            leg32 = Leg(f, cia, 1, torch, dev).run(steps, warmup)
             entry["fp32x"] = leg32.summary()
             entry["fp32x"]["speedup_vs_fp64"] = leg32.value / leg64.value
             entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), leg64.host_results())

@@ -636,11 +636,7 @@ struct BsDeriv {
     }
 };
 
-}  // namespace
-
-extern "C" {
-
-int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
+int fsg_ref_orbit_impl(const fsg_orbit_desc *d, fsg_orbit_out *o) {
     int rc = check_desc(d, 0.);
     if (rc) return rc;
     if (!o || !o->Zn) return set_error(FSG_E_INVALID, "bad orbit output");
@@ -715,7 +711,7 @@ int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
 }
 
 
-int fsg_ball_period(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int64_t *order) {
+int fsg_ball_period_impl(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int64_t *order) {
     int rc = check_desc(d, 8.);
     if (rc) return rc;
     if (!order || !(px_mant > 0.) || !std::isfinite(px_mant))
@@ -764,7 +760,7 @@ int fsg_ball_period(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int
     return 0;
 }
 
-int fsg_newton_eval(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fsg_mp_values *out) {
+int fsg_newton_eval_impl(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fsg_mp_values *out) {
     int rc = check_desc(d, 8.);
     if (rc) return rc;
     if (!out || order < 1 || out->cap < 0 || (out->cap > 0 && (!out->limbs || !out->neg || !out->exp)))
@@ -812,6 +808,33 @@ int fsg_newton_eval(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fs
     out->n = n;
     if (n > out->cap) return set_error(FSG_E_INVALID, "value buffer too small");
     return 0;
+}
+
+// C entry points: exceptions (allocation, thread creation) become error codes
+template <class F> int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(FSG_E_NOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return set_error(FSG_E_INVALID, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_ref_orbit(const fsg_orbit_desc *d, fsg_orbit_out *o) {
+    return guarded([&] { return fsg_ref_orbit_impl(d, o); });
+}
+
+int fsg_ball_period(const fsg_orbit_desc *d, double px_mant, int64_t px_exp, int64_t *order) {
+    return guarded([&] { return fsg_ball_period_impl(d, px_mant, px_exp, order); });
+}
+
+int fsg_newton_eval(const fsg_orbit_desc *d, int64_t order, int32_t divisors, fsg_mp_values *out) {
+    return guarded([&] { return fsg_newton_eval_impl(d, order, divisors, out); });
 }
 
 }  // extern "C"
