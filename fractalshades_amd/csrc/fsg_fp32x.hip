@@ -82,6 +82,28 @@ __device__ __forceinline__ bool xs_abs2_le(xs a, xs b) {
     return ldexp(ma, d) <= mb;
 }
 
+// lazily normalised intermediates of one step: products add exponents,
+// sums align on the larger one, nothing is renormalised until the step's
+// result (mantissas of a z^d + c step stay far inside the binary32 range:
+// at most d factors in [1, 2) and binomial coefficients <= 70).  Zero takes
+// an exponent low enough to align to nothing, small enough that the
+// exponents of the 8 zero factors a step can multiply still fit in int32.
+struct xt { float x, y; int32_t e; };
+constexpr int32_t XT_ZERO_E = -(1 << 25);
+
+__device__ __forceinline__ xt as_t(xs a) { return xt{a.x, a.y, a.e == XS_ZERO_E ? XT_ZERO_E : a.e}; }
+__device__ __forceinline__ xs of_t(xt a) { return xs_norm(a.x, a.y, a.e); }
+__device__ __forceinline__ xt operator+(xt a, xt b) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = max(a.e - E, -200), sb = max(b.e - E, -200);
+    const float x = ldexpf(a.x, sa) + ldexpf(b.x, sb), y = ldexpf(a.y, sa) + ldexpf(b.y, sb);
+    return xt{x, y, (x == 0.f && y == 0.f) ? XT_ZERO_E : E};
+}
+__device__ __forceinline__ xt operator*(xt a, xt b) {
+    return xt{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x, a.e + b.e};
+}
+__device__ __forceinline__ xt muls(double s, xt a) { const float f = (float)s; return xt{f * a.x, f * a.y, a.e}; }
+
 // ---------------------------------------------------------------------------
 // per-reference tables in the xs format, built once per context
 __global__ void k_xs_from_path(const double *m, const int32_t *e, int32_t n, xs *out) {
@@ -198,9 +220,10 @@ struct HoloF32 {
             cmul(s.z.x, s.z.y, sx + Zf_x, sy + Zf_y, px, py);         // z (z + 2 Z)
             s.z = xs_sum(px, py, s.z.e, s.c.x, s.c.y, s.c.e);
         } else {
-            if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? xs_zero() : DC[s.w]);
-            if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? DZ[0] : DZ[s.w]);
-            s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+            const xt z0 = as_t(s.z), Zt = as_t(Zr);
+            if (a.calc_dc) s.d = of_t(p_deriv<KIND, DEG>(z0, as_t(s.d), Zt, as_t(s.dyn ? xs_zero() : DC[s.w])));
+            if (a.calc_dz) s.h = of_t(p_deriv<KIND, DEG>(z0, as_t(s.h), Zt, as_t(s.nul ? DZ[0] : DZ[s.w])));
+            s.z = of_t(p_zn<KIND, DEG>(z0, Zt, as_t(s.c)));
         }
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
