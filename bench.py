@@ -159,6 +159,60 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
     return out
 
 
+def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
+    """A zoom sequence through the serving loop: n_frames frames, width
+    geometric from 4 down to 1e-2000, onto one 2100-digit nucleus whose orbit
+    is computed unwrapped to max_iter and kept resident (one base context per
+    arithmetic mode; every frame shares its orbit and BLA coefficients).  The
+    frames are submitted at once and return the nu / dem layers; binary64
+    and fp32x are run on the same frames and their escape counts compared."""
+    from fractalshades_amd import serving
+    from fractalshades_amd.postproc import post_desc
+    t0 = time.perf_counter()
+    centre = serving.movie_centre(max_iter)
+    orbit_s = time.perf_counter() - t0
+    specs = serving.movie_requests(n_frames, nx, ny, max_iter)
+    out = {"frames": n_frames, "shape": f"{nx}x{ny}", "max_iter": max_iter,
+           "widths": f"{specs[0].width} .. {specs[-1].width}", "orbit_points": max_iter + 1,
+           "orbit_host_s": round(orbit_s, 1)}
+    results = {}
+    for precision in ("fp64", "fp32x"):
+        t0 = time.perf_counter()
+        bases, reqs, posts = {}, [], []
+        for sp in specs:
+            f, cia, c_pix = serving.build_request(sp, [centre], precision)
+            posts.append(post_desc(f, "serve"))
+            key = cia.fields["xr_mode"]
+            if key not in bases:
+                bases[key] = cia.ctx
+            else:
+                cia = cia.shared_with(bases[key])
+            _ = cia.ctx
+            reqs.append((cia, c_pix))
+        prep_s = time.perf_counter() - t0
+        tiles = serving.layer_tiles(reqs)
+        srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
+        try:
+            rs = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts)
+        finally:
+            srv.close()
+        span = (max(r["t_done_ms"] for r in rs) - min(r["t_submit_ms"] for r in rs)) * 1e-3
+        pix_iters = sum(r["pixel_iters"] for r in rs)
+        results[precision] = [np.array(t.stop_iter, copy=True) for t in tiles]
+        out[precision] = {"host_prep_s": round(prep_s, 1), "makespan_s": span,
+                          "frames_per_s": n_frames / span, "pixel_iters": int(pix_iters),
+                          "pixel_iters_per_s": pix_iters / span}
+        for t in tiles:
+            t.close()
+        for cia, _ in reqs:
+            cia.close()
+    mis = sum(int(np.count_nonzero(a != b)) for a, b in zip(results["fp64"], results["fp32x"]))
+    npx = sum(a.size for a in results["fp64"])
+    out["fp32x"]["speedup_vs_fp64"] = out["fp64"]["makespan_s"] / out["fp32x"]["makespan_s"]
+    out["fp32x"]["escape_count_mismatch_frac"] = mis / npx
+    return out
+
+
 class Leg:
     """One workload x precision on the GPU: device buffers + timed steps."""
 
@@ -243,6 +297,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
     ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--movie", type=int, default=64, help="frames of the zoom-sequence leg (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (PMC counter calibration)")
     args = ap.parse_args()
@@ -337,6 +392,8 @@ def main():
         lat512 = serve["fp64_layers"]["poisson"]["latency_512sq"]
         if lat512:
             out["tile_latency_512sq_ms"] = {"p50": lat512["p50_ms"], "p99": lat512["p99_ms"]}
+    if args.movie > 0:
+        out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=960, ny=540)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
     print(json.dumps(out), flush=True)

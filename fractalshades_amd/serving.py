@@ -245,6 +245,46 @@ def serve_centres(n_centres=8, digits=2100, cache=True):
     return centres
 
 
+def movie_centre(max_iter, centre=0, digits=2100):
+    """The zoom sequence's centre: a cached nucleus with its orbit computed
+    unwrapped up to max_iter (one resident orbit of max_iter + 1 points for
+    every frame), binary64 and Xrange registers."""
+    from . import models
+    nuc, _ = _cached_nuclei(digits)[centre]
+    prec = mpmath.libmp.dps_to_prec(digits)
+    orbits = {}
+    for xr_mode in (False, True):
+        f = models.Perturbation_mandelbrot("/tmp/fractalshades_amd")
+        f.prec = prec
+        f.M_divergence = 1.e3
+        f.dx = mpmath.mpf("1e-400") if xr_mode else mpmath.mpf("1e-100")
+        with mpmath.workprec(prec):
+            div_iter, Zn_path, xr = f.FP_loop(max_iter + 1, nuc)
+        orbits[xr_mode] = (Zn_path, div_iter, xr)
+    return Centre(nuc, None, digits, orbits)
+
+
+def movie_requests(n_frames, nx=480, ny=270, max_iter=500000, log10_width=(math.log10(4.), -2000.)):
+    """A zoom sequence onto the centre: n_frames frames of nx x ny pixels,
+    width geometric from 10^log10_width[0] down to 10^log10_width[1]."""
+    out = []
+    for k in range(n_frames):
+        lw = log10_width[0] + (log10_width[1] - log10_width[0]) * k / max(n_frames - 1, 1)
+        mant = 10 ** (lw - math.floor(lw))
+        out.append(RequestSpec(0, nx, ny, f"{mant:.6f}e{int(math.floor(lw))}", max_iter, 0., 0.))
+    return out
+
+
+def _cached_nuclei(digits):
+    with open(SERVE_CENTRES_FILE) as fh:
+        d = json.load(fh)
+    if d.get("digits") != digits:
+        raise ValueError(f"no cached nuclei at {digits} digits")
+    prec = mpmath.libmp.dps_to_prec(digits)
+    with mpmath.workprec(prec):
+        return [(mpmath.mpc(c["x"], c["y"]), c["order"]) for c in d["centres"]]
+
+
 @dataclass
 class RequestSpec:
     centre: int
