@@ -65,6 +65,7 @@ class RefDesc(ctypes.Structure):
         ("calc_dzndc", ctypes.c_int32), ("interior_detect", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("calc_orbit", ctypes.c_int32),
         ("backshift", ctypes.c_int32), ("pad2", ctypes.c_int32),
+        ("deriv_mant", ctypes.c_double), ("deriv_exp", ctypes.c_int32), ("pad3", ctypes.c_int32),
     ]
 
 
@@ -193,7 +194,7 @@ def load(path=None):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.fsg_abi_version() != 2:
+        if lib.fsg_abi_version() != 3:
             raise FsgError("libfsgpu.so ABI version mismatch")
         if path is None:
             _lib = lib

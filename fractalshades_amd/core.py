@@ -162,6 +162,7 @@ class Fractal:
             "zoom_kwargs": dict(getattr(self, "zoom_kwargs", {})),
             "state": state,
             "cycle_indep_args": cycle_indep_args,
+            "iterate": iterate,
         }
 
     def get_cycling_dep_args(self, calc_name, chunk_slice, jitter=False, supersampling=None):

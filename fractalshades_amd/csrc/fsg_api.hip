@@ -96,12 +96,17 @@ struct HostRef {
     }
 };
 
+// the scale of the dz/dc paths: dx, times the projection's scale
+xf deriv_scale(const fsg_ref_desc *d) {
+    return d->deriv_mant != 0. ? mkf(d->deriv_mant, d->deriv_exp) : mkf(d->dx_mant, d->dx_exp);
+}
+
 // dZndc: D_0 = 0, D_i = f'(Z_{i-1}) D_{i-1} + s, wrapped value at [0] for cycles
 void holo_dzndc(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e) {
     const fsg_ref_desc *d = h.d;
     const int64_t L = h.L, valid = std::min<int64_t>(h.L, h.div_iter);
     m.assign(L, mk(0., 0.)); e.assign(L, 0);
-    const xf s = mkf(d->dx_mant, d->dx_exp);
+    const xf s = deriv_scale(d);
     if (!d->xr_mode) {
         const double ss = to_std(s);
         for (int64_t i = 1; i < valid; i++) m[i] = dfdz_rt(d->kind, d->degree, h.Z(i - 1)) * m[i - 1] + mk(ss, 0.);
@@ -142,7 +147,7 @@ void bs_paths(const HostRef &h, std::vector<double> P[4], std::vector<int32_t> E
     const int64_t L = h.L, valid = std::min<int64_t>(h.L, h.div_iter);
     for (int q = 0; q < 4; q++) { P[q].assign(L, 0.); E[q].assign(L, 0); }
     if (!d->calc_dzndc) return;
-    const xf s = mkf(d->dx_mant, d->dx_exp);
+    const xf s = deriv_scale(d);
     auto incr_std = [&](int64_t from, int64_t to) {
         double J[4];
         cplx R = h.Z(from);

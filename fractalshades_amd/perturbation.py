@@ -12,9 +12,10 @@ import logging
 import mpmath
 import numpy as np
 
+from . import projection as fsproj
 from . import settings
 from . import mpmath_loop as fsFP
-from ._native import GpuContext, FSG_PREC_FP64, FSG_PREC_FP32X
+from ._native import GpuContext, FSG_KIND_BS, FSG_PREC_FP64, FSG_PREC_FP32X
 from .core import Fractal
 from .utils import zoom_options
 from .xrange import mpf_to_xr, mpc_to_xr, xr_to_float
@@ -30,12 +31,13 @@ class CycleIndepArgs:
     engine descriptor (fields + arrays) and, created on first use, the
     device-resident context built from it."""
 
-    def __init__(self, fields, arrays, interrupted, share=None):
+    def __init__(self, fields, arrays, interrupted, share=None, projection=None):
         self.fields = fields
         self.arrays = arrays
         self.interrupted = interrupted
         self.last_stats = None
         self.share = share      # GpuContext whose orbit / BLA coefficients to reuse
+        self.projection = projection   # non-identity screen projection, applied on the host
         self._ctx = None
 
     @property
@@ -51,12 +53,23 @@ class CycleIndepArgs:
 
     def with_fields(self, **over):
         """A copy with some descriptor fields replaced (its own context)."""
-        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted, self.share)
+        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted, self.share,
+                              self.projection)
 
     def shared_with(self, base_ctx):
         """A copy whose context reuses base_ctx's device orbit and BLA
         coefficients (same orbit required)."""
-        return CycleIndepArgs(self.fields, self.arrays, self.interrupted, base_ctx)
+        return CycleIndepArgs(self.fields, self.arrays, self.interrupted, base_ctx, self.projection)
+
+    def dzndc_rows(self):
+        """Rows of the raw Z array that hold dz/dc (the burning ship's four
+        Jacobian rows): zn, [dzndz], dzndc, [zn_orbit] / xn, yn, 4 x J."""
+        f = self.fields
+        if not f["calc_dzndc"]:
+            return []
+        if f["kind"] == FSG_KIND_BS:
+            return [2, 3, 4, 5]
+        return [1 + int(bool(f["interior_detect"]))]
 
 
 class PerturbationFractal(Fractal):
@@ -80,17 +93,24 @@ class PerturbationFractal(Fractal):
             self.x = mpmath.mpf(x)
             self.y = mpmath.mpf(y)
             self.dx = dx = mpmath.mpf(dx)
-        if projection not in ("cartesian", None) and type(projection).__name__ != "Cartesian":
-            raise NotImplementedError("only the Cartesian projection is supported")
+        # a str selects the Cartesian projection, as the reference's
+        # backward-compatible path does (perturbation.py:95-104)
+        if projection is None or isinstance(projection, str):
+            projection = fsproj.Cartesian()
+        if not isinstance(projection, fsproj.Projection):
+            raise TypeError("projection must be a fractalshades_amd.projection.Projection")
+        self.projection = projection
         self._skew = None
         if has_skew:
             self._skew = np.array(((skew_00, skew_01), (skew_10, skew_11)), dtype=np.float64)
         self.dx_std = float(dx)
         self.dx_xr = mpf_to_xr(dx)
         self.lin_mat = self.get_lin_mat()
-        # precision check (perturbation.py:123-135), Cartesian min_local_scale = 1
+        projection.adjust_to_zoom(self)
+        nx = self.nx
+        # precision check (perturbation.py:123-135)
         with mpmath.workdps(6):
-            pix = dx / nx
+            pix = projection.min_local_scale * dx / nx
             required_dps = int(-mpmath.log10(pix / nx) + 1)
         if required_dps > precision:
             raise ValueError(
@@ -224,7 +244,7 @@ class PerturbationFractal(Fractal):
     def ref_point_kc(self):
         """Bound of |c - ref| over the frame corners, x 1.1 (perturbation.py:166-194)."""
         with mpmath.workprec(self.prec):
-            w, h = 1., 1. / self.xy_ratio
+            w, h = self.projection.bounding_box(self.xy_ratio)
             mat = self.lin_mat
             ref = self.FP_params["ref_point"]
             c0 = self.x + 1j * self.y
@@ -305,7 +325,13 @@ class PerturbationFractal(Fractal):
             "precision": int(spec.get("precision", PRECISIONS[settings.gpu_precision])),
             "calc_orbit": int(bool(spec.get("calc_orbit", False))),
             "backshift": int(spec.get("backshift", 0) or 0) if spec.get("calc_orbit") else 0,
+            "deriv_mant": 0., "deriv_exp": 0,
         }
+        scale = self.projection.scale
+        if scale != 1.:
+            # derivative paths at dx * projection.scale (perturbation.py:459-467)
+            with mpmath.workprec(self.prec):
+                fields["deriv_mant"], fields["deriv_exp"] = mpf_to_xr(self.dx * scale)
         arrays = {
             "Zn_path": pd["Zn_path"].view(np.float64),
             "xr_index": pd["xr_index"], "xr_mant": pd["xr_mant"], "xr_exp": pd["xr_exp"],
@@ -319,7 +345,8 @@ class PerturbationFractal(Fractal):
         self.get_FP_orbit()
         fields, arrays = self.get_ref_desc(spec)
         self.kc = (fields["kc_mant"], fields["kc_exp"])
-        return CycleIndepArgs(fields, arrays, self._interrupted)
+        proj = None if fsproj.is_cartesian(self.projection) else self.projection
+        return CycleIndepArgs(fields, arrays, self._interrupted, projection=proj)
 
     @staticmethod
     def numba_cycle_call(cycle_dep_args, cycle_indep_args):
@@ -327,9 +354,79 @@ class PerturbationFractal(Fractal):
         (perturbation.py:414-428)."""
         c_pix, Z, U, stop_reason, stop_iter = cycle_dep_args
         args = cycle_indep_args
-        rc, stats = args.ctx.tile(c_pix, Z, U, stop_reason, stop_iter, args.interrupted)
+        proj = args.projection
+        c_in = c_pix if proj is None else np.ascontiguousarray(proj.f(c_pix))
+        rc, stats = args.ctx.tile(c_in, Z, U, stop_reason, stop_iter, args.interrupted)
         args.last_stats = stats
+        if proj is not None and proj.dzndc_modifier is not None:
+            # perturbation.py:1387-1388, 1773-1777
+            mod = proj.dzndc_modifier(c_pix)
+            for row in args.dzndc_rows():
+                Z[row] *= mod
         return rc
+
+    def calc_raw_by_steps(self, calc_name, exp_zoom_step=None, keep_bands=False):
+        """An exponential map computed by bands of h, each with the BLA
+        validity radii and the derivative scale of its own |c| bound, as the
+        reference's save_expdb_by_steps / reset_bla_tree do
+        (core.py:891-963, perturbation.py:565-580).  Every band shares the
+        device orbit and BLA coefficients of the first (only the radii are
+        rebuilt) and runs its tiles in one launch.  Returns calc_raw's dict
+        (whole-frame arrays, row-major (y, x)); with keep_bands, also
+        "bands": [(CycleIndepArgs, chunk slices)]."""
+        proj = self.projection
+        if not isinstance(proj, fsproj.Expmap):
+            raise ValueError("steps apply to the Expmap projection")
+        cd = self._calc_data[calc_name]
+        state = cd["state"]
+        if getattr(state, "subset", None) is not None:
+            raise NotImplementedError("steps with a subset")
+        base = cd["cycle_indep_args"]
+        nh = proj.nh(self)
+        stp = int(exp_zoom_step or proj.nt(self))
+        cs = settings.chunk_size
+        nx, ny = self.nx, self.ny
+        n_Z, n_U = len(state.codes[0]), len(state.codes[1])
+        Z = np.zeros((n_Z, ny, nx), dtype=state.complex_type)
+        U = np.zeros((n_U, ny, nx), dtype=self.int_type)
+        sr = -np.ones((1, ny, nx), dtype=self.termination_type)
+        si = np.zeros((1, ny, nx), dtype=self.int_type)
+        bands = []
+        try:
+            for r in range(0, nh + 1, stp):
+                i_max = min(r + stp, nh)
+                i_min = max(r - cs, 0)
+                proj.set_exp_zoom_step((proj.hmax * i_max + proj.hmin * (nh - i_max)) / nh,
+                                       (proj.hmax * i_min + proj.hmin * (nh - i_min)) / nh)
+                idx = 3 if proj.premul_1j else 1         # iyy (vertical) or ixx (horizontal)
+                chunks = [c for c in self.chunk_slices() if r < c[idx] <= r + stp]
+                if not chunks:
+                    continue
+                fields, arrays = self.get_ref_desc(cd["iterate"])
+                args = CycleIndepArgs(fields, arrays, self._interrupted, projection=proj).shared_with(base.ctx)
+                args.step_h = (proj.exp_step_hmax, proj.exp_step_hmin)
+                c_pix = np.concatenate([np.ravel(self.chunk_pixel_pos(c)) for c in chunks])
+                res = self._result_arrays(state, c_pix)
+                if self.numba_cycle_call(res, args) != 0:
+                    raise RuntimeError("calculation interrupted")
+                o = 0
+                for ix, ixx, iy, iyy in chunks:
+                    k = (ixx - ix) * (iyy - iy)
+                    for dst, src in zip((Z, U, sr, si), res[1:]):
+                        dst[:, iy:iyy, ix:ixx] = src[:, o:o + k].reshape(-1, iyy - iy, ixx - ix)
+                    o += k
+                if keep_bands:
+                    bands.append((args, chunks))
+                else:
+                    args.close()
+        finally:
+            proj.del_exp_zoom_step()
+        out = {"c_pix": np.ravel(self.chunk_pixel_pos((0, nx, 0, ny))),
+               "Z": Z.reshape(n_Z, -1), "U": U.reshape(n_U, -1), "stop_reason": sr.reshape(1, -1),
+               "stop_iter": si.reshape(1, -1), "codes": state.codes}
+        if keep_bands:
+            out["bands"] = bands
+        return out
 
     def calc_raw(self, calc_name, jitter=False, supersampling=None):
         args = self._calc_data[calc_name]["cycle_indep_args"]

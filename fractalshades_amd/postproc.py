@@ -123,6 +123,11 @@ def post_desc(f, calc_name, floor_iter=0., px_snap=None):
     them, postproc.py:320-347), the row of dz/dc (dX/da for the burning
     ship) in its raw Z array, and the frame's skew for the normal map."""
     from ._native import PostDesc
+    from .projection import is_cartesian
+    if not is_cartesian(getattr(f, "projection", None)):
+        # the reference's layers rotate dz/dc with the projection's df
+        # (postproc.py:184-206), which the device layers do not
+        raise NotImplementedError("device layers of a non-Cartesian projection")
     state = f._calc_data[calc_name]["state"]
     names = list(state.codes[0])
     holo = bool(getattr(f, "holomorphic", True))

@@ -1,0 +1,162 @@
+"""Projections of the screen pixels before iteration
+(src/fractalshades/projection.py): the Cartesian identity and the
+exponential map.
+
+A projection maps a screen pixel z_pix (x in [-0.5, 0.5], y scaled by
+1 / xy_ratio) to f(z_pix); the perturbation loop then takes
+c = dx * lin_mat . f(z_pix) + drift (ref_path_c_from_pix,
+perturbation.py:2214-2230).  The derivative paths are scaled by
+dx * projection.scale (perturbation.py:459-467) and, after the loop, dz/dc
+is multiplied by the projection's ``dzndc_modifier`` of the pixel
+(perturbation.py:1387-1388, 1773-1777).
+
+Here f is applied to the pixel offsets on the host with numpy, before they
+go to the GPU, and the modifier to the results after; the kernels only ever
+see the projected offsets, so the GPU and the oracle consume the same
+inputs.  (The reference evaluates f inside its numba loop: numba's complex
+``exp`` and numpy's may differ in the last bit of some pixels.)
+"""
+import inspect
+
+import mpmath
+import numpy as np
+
+from . import settings
+
+
+class Projection:
+    """A mapping of the screen pixels (projection.py:13-210)."""
+
+    scale = 1.0
+
+    @property
+    def init_kwargs(self):
+        return {name: getattr(self, name) for name in inspect.signature(self.__init__).parameters}
+
+    def __eq__(self, other):
+        return other.__class__ == self.__class__ and other.init_kwargs == self.init_kwargs
+
+    def __hash__(self):
+        return hash(self.__class__.__name__)
+
+    def adjust_to_zoom(self, fractal):
+        """Constraints the projection puts on the zoom (none by default)."""
+
+    def f(self, pix):
+        return pix
+
+    dzndc_modifier = None
+
+    def bounding_box(self, xy_ratio):
+        raise NotImplementedError
+
+    @property
+    def min_local_scale(self):
+        raise NotImplementedError
+
+
+class Cartesian(Projection):
+    """The identity (projection.py:212-266)."""
+
+    def __init__(self, expmap_seam=None):
+        self.scale = 1.0
+        self.expmap_seam = expmap_seam
+        if expmap_seam is not None:
+            seam = float(expmap_seam)
+            # projection.py:250-262
+            self.dzndc_modifier = lambda cpix: np.abs(cpix + 1.e-6) * seam
+
+    def bounding_box(self, xy_ratio):
+        return 1., 1. / xy_ratio
+
+    @property
+    def min_local_scale(self):
+        return 1.
+
+
+class Expmap(Projection):
+    """Exponential map z_pix -> exp(h_moy + dh z_pix), h from hmin to hmax
+    along the screen's x axis ("horizontal") or y axis ("vertical")
+    (projection.py:226-500, without the stepwise mode of movie databases).
+    h is kept in binary64, so exp(hmax) must stay below 1e300 (the
+    reference switches h to Xrange above that)."""
+
+    def __init__(self, hmin, hmax, rotates_df=True, orientation="horizontal"):
+        if not 0 <= hmin < hmax:
+            raise ValueError("Provide hmin, hmax with:  0 <= hmin < hmax for Expmap")
+        if mpmath.exp(hmax) > 1. / settings.xrange_zoom_level:
+            raise NotImplementedError("Expmap with exp(hmax) above 1e300 (Xrange h)")
+        self.hmin = hmin
+        self.hmax = hmax
+        self.rotates_df = rotates_df
+        self.orientation = orientation
+        self.premul_1j = {"horizontal": False, "vertical": True}[orientation]
+        self.hmoy = 0.5 * (float(hmin) + float(hmax))
+        self.dh = float(hmax) - float(hmin)
+        self.xy_ratio = None
+        self.use_step = False
+
+    def set_exp_zoom_step(self, exp_step_hmax, exp_step_hmin):
+        """Compute the next band of h only (projection.py:311-331): the
+        derivative scale, the modifier's shift and the bounding box follow
+        the band; f does not change."""
+        self.exp_step_hmax = float(exp_step_hmax)
+        self.exp_step_hmin = float(exp_step_hmin)
+        self.exp_step_hmoy = 0.5 * (self.exp_step_hmax + self.exp_step_hmin)
+        self.exp_step_dh = self.exp_step_hmax - self.exp_step_hmin
+        self.use_step = True
+
+    def del_exp_zoom_step(self):
+        self.use_step = False
+
+    def nh(self, fractal):
+        return fractal.ny if self.premul_1j else fractal.nx
+
+    def nt(self, fractal):
+        return fractal.nx if self.premul_1j else fractal.ny
+
+    @property
+    def pix_to_ht(self):
+        return (1j * self.dh * self.xy_ratio) if self.premul_1j else self.dh
+
+    def adjust_to_zoom(self, fractal):
+        """The screen's t axis spans 2 pi: xy_ratio (and nx in the vertical
+        orientation) follow from dh (projection.py:339-360)."""
+        if self.premul_1j:
+            xy_ratio = (np.pi * 2.) / self.dh
+            nx = int(fractal.nx * xy_ratio + 0.5)
+        else:
+            xy_ratio = self.dh / (np.pi * 2.)
+            nx = fractal.nx
+        fractal.xy_ratio = self.xy_ratio = xy_ratio
+        fractal.nx = nx
+        if hasattr(fractal, "zoom_kwargs"):
+            fractal.zoom_kwargs["xy_ratio"] = xy_ratio
+            fractal.zoom_kwargs["nx"] = nx
+
+    def f(self, pix):
+        """exp(h_moy + pix_to_ht * pix) (projection.py:363-372)."""
+        return np.exp(self.hmoy + self.pix_to_ht * np.asarray(pix, dtype=np.complex128))
+
+    def dzndc_modifier(self, cpix):
+        """exp(Re(pix_to_ht * cpix) + h_shift), h_shift = h_moy, or
+        h_moy - the band's h_moy in steps (projection.py:454-471)."""
+        hshift = (self.hmoy - self.exp_step_hmoy) if self.use_step else self.hmoy
+        return np.exp(np.real(self.pix_to_ht * np.asarray(cpix, dtype=np.complex128)) + hshift)
+
+    @property
+    def scale(self):
+        return mpmath.exp(self.exp_step_hmoy if self.use_step else self.hmoy)
+
+    def bounding_box(self, xy_ratio):
+        w = mpmath.exp(self.exp_step_hmax if self.use_step else self.hmax)
+        return w, w
+
+    @property
+    def min_local_scale(self):
+        return mpmath.exp(self.hmin)
+
+
+def is_cartesian(projection):
+    return projection is None or isinstance(projection, str) or (
+        isinstance(projection, Cartesian) and projection.dzndc_modifier is None)

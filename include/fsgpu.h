@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 2
+#define FSG_ABI_VERSION 3
 
 /* model kinds */
 #define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
@@ -90,6 +90,9 @@ typedef struct fsg_ref_desc {
                                 or two float64 fields xn_orbit, yn_orbit (burning ship); FP64 only */
     int32_t backshift;       /* > 0 when calc_orbit */
     int32_t pad2;
+    double deriv_mant;       /* scale of the dz/dc paths, dx * projection.scale (scale_deriv_xr, */
+    int32_t deriv_exp;       /*   perturbation.py:459-467); deriv_mant == 0: dx */
+    int32_t pad3;
 } fsg_ref_desc;
 
 /* plain / BLA step counters of the last call, for the FLOP accounting */

@@ -274,7 +274,8 @@ typedef struct {
     const int32_t *xr_idx;       /* [n_xr] sorted */
     const double *xr_m;          /* holo: [2 n_xr] complex; BS: [2 n_xr] (x, y) */
     const int32_t *xr_e;         /* holo: [n_xr]; BS: [2 n_xr] (x, y) */
-    double dx_m; int32_t dx_e;   /* dx_xr == lin_scale_xr == scale_deriv_xr */
+    double dx_m; int32_t dx_e;   /* dx_xr == lin_scale_xr */
+    double sd_m; int32_t sd_e;   /* scale_deriv_xr = dx_xr * projection.scale (perturbation.py:459-467) */
     double drift_m[2]; int32_t drift_e[2];
     double lin_mat[4];
     double kc_m; int32_t kc_e;
@@ -376,7 +377,7 @@ static inline xrc ref_xr(const or_ctx *k, int64_t i, int *is_xr) {
 static int holo_paths(or_ctx *k) {
     int64_t L = k->L;
     int64_t valid = L < k->div_iter ? L : k->div_iter;
-    xrf scale = XF(k->dx_m, k->dx_e);
+    xrf scale = XF(k->sd_m, k->sd_e);
     double scale_std = xrf_std(scale);
     int dummy;
     if (k->calc_dzndc) {
@@ -446,7 +447,7 @@ static int holo_paths(or_ctx *k) {
 static int bs_paths(or_ctx *k) {
     int64_t L = k->L;
     int64_t valid = L < k->div_iter ? L : k->div_iter;
-    xrf scale = XF(k->dx_m, k->dx_e);
+    xrf scale = XF(k->sd_m, k->sd_e);
     double s = xrf_std(scale);
     for (int q = 0; q < 4; q++) {
         k->dbs_m[q] = (double *)calloc(L, sizeof(double));
@@ -1371,6 +1372,7 @@ typedef struct {
     double M2, eps2;
     int32_t calc_dzndc, calc_dzndz;
     int32_t calc_orbit, backshift;
+    double sd_m; int32_t sd_e; int32_t pad3;   /* derivative scale; sd_m == 0: dx */
 } or_desc;
 
 void *or_ctx_create(const or_desc *d) {
@@ -1379,6 +1381,7 @@ void *or_ctx_create(const or_desc *d) {
     k->L = d->L; k->div_iter = d->div_iter; k->order = d->order;
     k->Zn = d->Zn; k->n_xr = d->n_xr; k->xr_idx = d->xr_idx; k->xr_m = d->xr_m; k->xr_e = d->xr_e;
     k->dx_m = d->dx_m; k->dx_e = d->dx_e;
+    k->sd_m = d->sd_m != 0. ? d->sd_m : d->dx_m; k->sd_e = d->sd_m != 0. ? d->sd_e : d->dx_e;
     k->drift_m[0] = d->drift_m[0]; k->drift_m[1] = d->drift_m[1];
     k->drift_e[0] = d->drift_e[0]; k->drift_e[1] = d->drift_e[1];
     memcpy(k->lin_mat, d->lin_mat, sizeof(k->lin_mat));

@@ -36,6 +36,7 @@ class _Desc(ctypes.Structure):
         ("M2", ctypes.c_double), ("eps2", ctypes.c_double),
         ("calc_dzndc", ctypes.c_int32), ("calc_dzndz", ctypes.c_int32),
         ("calc_orbit", ctypes.c_int32), ("backshift", ctypes.c_int32),
+        ("sd_m", ctypes.c_double), ("sd_e", ctypes.c_int32), ("pad3", ctypes.c_int32),
     ]
 
 
@@ -99,6 +100,7 @@ class OracleCtx:
         d.calc_dzndc, d.calc_dzndz = fields["calc_dzndc"], fields["interior_detect"]
         d.calc_orbit = int(fields.get("calc_orbit", 0))
         d.backshift = int(fields.get("backshift", 0))
+        d.sd_m, d.sd_e = float(fields.get("deriv_mant", 0.)), int(fields.get("deriv_exp", 0))
         self.fields = fields
         self.kind = fields["kind"]
         self._h = lib().or_ctx_create(ctypes.byref(d))

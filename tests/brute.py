@@ -1,12 +1,18 @@
 """Brute-force ground truth: direct arbitrary-precision iteration of a pixel,
 without perturbation (mpmath)."""
 import mpmath
+import numpy as np
 
 
 def pixel_c(f, pix):
-    """centre + dx * (lin_mat . pix): rotation and skew included
-    (lin_proj_impl, perturbation.py:2665-2672)."""
+    """centre + dx * (lin_mat . f(pix)): rotation and skew included
+    (lin_proj_impl, perturbation.py:2665-2672); f the projection, rounded to
+    complex128 as the reference's loop has it (the escape count of a pixel
+    near the boundary follows that rounding)."""
     m = f.lin_mat
+    proj = getattr(f, "projection", None)
+    if proj is not None and type(proj).__name__ != "Cartesian":
+        pix = complex(proj.f(np.array([pix]))[0])
     px, py = float(pix.real), float(pix.imag)
     with mpmath.workprec(f.prec + 64):
         ox = mpmath.mpf(float(m[0, 0])) * px + mpmath.mpf(float(m[0, 1])) * py

@@ -88,3 +88,26 @@ def frame_inputs(f, calc_name="t"):
     args = f._calc_data[calc_name]["cycle_indep_args"]
     c_pix = np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny)))
     return args, np.ascontiguousarray(c_pix)
+
+
+def expmap_frame(name="M2_E20", nx=60, hmin=0., hmax=10., dx=None, orientation="horizontal", **calc_over):
+    """A test frame's location seen through the exponential map
+    exp(h + i t), h in [hmin, hmax] (fractalshades_amd.projection.Expmap)."""
+    from fractalshades_amd.projection import Expmap
+    model, zoom_kw, calc_kw = FRAMES[name]
+    zoom_kw = dict(zoom_kw, projection=Expmap(hmin, hmax, orientation=orientation))
+    if dx is not None:
+        zoom_kw["dx"] = dx
+    if model == "M2":
+        f = fsa.Perturbation_mandelbrot("/tmp/fsa_test")
+    else:
+        f = fsa.Perturbation_burning_ship("/tmp/fsa_test", flavor=FRAMES[name + "_flavor"])
+    f.zoom(nx=nx, xy_ratio=1.0, theta_deg=0., **zoom_kw)
+    kw = dict(calc_name="t", subset=None, M_divergence=1.e3, epsilon_stationnary=1.e-3,
+              BLA_eps=1.e-6)
+    if model == "BS":
+        kw.pop("epsilon_stationnary")
+    kw.update(calc_kw)
+    kw.update(calc_over)
+    f.calc_std_div(**kw)
+    return f

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 def with_fields(args, **over):
-    return CycleIndepArgs(dict(args.fields, **over), args.arrays, None)
+    return CycleIndepArgs(dict(args.fields, **over), args.arrays, None, projection=args.projection)
 
 
 @pytest.mark.parametrize("name", ["M2_E20", "M2_E11", "M2_divref", "M2_divref5", "M2_int32"])
@@ -190,3 +190,60 @@ def test_tables_global_and_lds_bitexact(case, monkeypatch):
             monkeypatch.delenv("FSG_TABLES", raising=False)
         got, _ = run_gpu(with_fields(args), c_pix)
         assert_same(got, ref)
+
+
+@pytest.mark.parametrize("name,orientation,xr", [("M2_E20", "horizontal", 0), ("M2_E20", "vertical", 1),
+                                                 ("M2_E11", "horizontal", 0), ("BS_ex14", "horizontal", 0),
+                                                 ("BS_ex14", "vertical", 1)])
+def test_expmap_bitexact(name, orientation, xr):
+    """The exponential map through the product path (numba_cycle_call:
+    pixel offsets projected on the host, the GPU loop, dz/dc times the
+    projection's modifier) against the oracle on the same projected offsets
+    with the same modifier."""
+    kw = {"max_iter": 6000} if name.startswith("BS") else {}
+    f = frames.expmap_frame(name, nx=48, hmin=0., hmax=10., orientation=orientation, **kw)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=xr)
+    assert args.projection is f.projection
+    print(f"case expmap {name} {orientation} xr={xr}", flush=True)
+    state = f._calc_data["t"]["state"]
+    got = f._result_arrays(state, c_pix)
+    assert f.numba_cycle_call(got, args) == 0
+    cp = np.ascontiguousarray(f.projection.f(c_pix))
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(cp)
+    mod = f.projection.dzndc_modifier(c_pix)
+    for row in args.dzndc_rows():
+        ref[0][row] *= mod
+    assert_same(got[1:], ref)
+
+
+@pytest.mark.parametrize("orientation", ["horizontal", "vertical"])
+def test_expmap_steps_bitexact(orientation, monkeypatch):
+    """The exponential map by bands of h (calc_raw_by_steps: per band, the
+    BLA radii and derivative scale of its |c| bound, on the shared device
+    orbit): every band's tiles equal the oracle run with that band's
+    descriptor and modifier."""
+    from fractalshades_amd import settings
+    monkeypatch.setattr(settings, "chunk_size", 16)
+    f = frames.expmap_frame("M2_E11", nx=48, hmin=0., hmax=10., orientation=orientation)
+    print(f"case expmap steps {orientation}", flush=True)
+    res = f.calc_raw_by_steps("t", exp_zoom_step=16, keep_bands=True)
+    assert len(res["bands"]) >= 3
+    nx, ny = f.nx, f.ny
+    proj = f.projection
+    kcs = set()
+    for r, (args, chunks) in enumerate(res["bands"]):
+        kcs.add((args.fields["kc_mant"], args.fields["kc_exp"]))
+        for ix, ixx, iy, iyy in chunks:
+            c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((ix, ixx, iy, iyy))))
+            ref = orc.OracleCtx(args.fields, args.arrays).cycles(np.ascontiguousarray(proj.f(c_pix)))
+            # the band's modifier (the projection is back out of step mode)
+            hshift = proj.hmoy - 0.5 * (args.step_h[0] + args.step_h[1])
+            mod = np.exp(np.real(proj.pix_to_ht * c_pix) + hshift)
+            for row in args.dzndc_rows():
+                ref[0][row] *= mod
+            sel = (slice(None), slice(iy, iyy), slice(ix, ixx))
+            got = tuple(np.ascontiguousarray(a.reshape(a.shape[0], ny, nx)[sel]).reshape(a.shape[0], -1)
+                        for a in (res["Z"], res["U"], res["stop_reason"], res["stop_iter"]))
+            assert_same(got, ref)
+    assert len(kcs) >= 3           # each band has its own BLA bound

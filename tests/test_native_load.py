@@ -23,7 +23,7 @@ def test_library_exports_all_declared():
     lib = native.load()
     for s in declared_symbols():
         assert hasattr(lib, s), s
-    assert lib.fsg_abi_version() == 2
+    assert lib.fsg_abi_version() == 3
     assert lib.fsg_work_bytes() >= 40
 
 
@@ -35,4 +35,5 @@ def test_desc_layout_matches_header():
     assert d.lin_mat.offset == 120
     assert d.precision.offset == 208
     assert d.calc_orbit.offset == 212 and d.backshift.offset == 216
-    assert native.ctypes.sizeof(d) == 224
+    assert d.deriv_mant.offset == 224 and d.deriv_exp.offset == 232
+    assert native.ctypes.sizeof(d) == 240

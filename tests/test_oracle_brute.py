@@ -1,6 +1,7 @@
 """Pins the oracle's perturbation loop (std and Xrange modes, BLA on and off)
 against brute-force direct iteration in mpmath on sampled pixels of the
 reference test suite's zoom locations."""
+import mpmath
 import numpy as np
 import pytest
 
@@ -158,3 +159,49 @@ def test_bs_vs_brute_force(name):
         same_n += int(nb == si[0, i])
         close += int(nb == si[0, i] and abs(complex(Z[0, i], Z[1, i]) - zb) <= 1e-8 * abs(zb))
     assert same_n >= 9 and close >= 6, (same_n, close)
+
+
+@pytest.mark.parametrize("orientation", ["horizontal", "vertical"])
+def test_expmap_vs_brute_force(orientation):
+    """The exponential map (projection.Expmap, h in [0, 10]): the pixel
+    offsets projected on the host (exp(h_moy + pix_to_ht pix), checked
+    against mpmath to binary64 accuracy) and iterated by the oracle (BLA off)
+    escape at the brute-force iteration of c = centre + dx f(pix) (but for
+    pixels shadowing the reference cycle, where binary64 rounding decides),
+    z_n within 1e-6; dz/dc is scaled by dx exp(h_moy), as the reference
+    scales the derivative paths."""
+    f = frames.expmap_frame("M2_E20", nx=40, hmin=0., hmax=10., dx="1e-24", orientation=orientation)
+    args, c_pix = frames.frame_inputs(f)
+    assert args.projection is f.projection
+    if orientation == "horizontal":
+        assert f.nx == 40 and abs(f.xy_ratio - 10. / (2 * np.pi)) < 1e-15
+    else:
+        assert abs(f.xy_ratio - 2 * np.pi / 10.) < 1e-15 and f.nx == int(40 * f.xy_ratio + 0.5)
+    cp = np.ascontiguousarray(f.projection.f(c_pix))
+    off = orc.OracleCtx(dict(args.fields, bla_on=0), args.arrays).cycles(cp)
+    scale = float(mpmath.exp(f.projection.hmoy))
+    sel = sample(len(c_pix), 16)
+    p = f.projection
+    same_n = checked = 0
+    for i in sel:
+        # the projected offset: exp of the binary64 argument h_moy + pix_to_ht pix
+        # (as the reference forms it) to binary64 accuracy
+        arg = complex(p.hmoy + p.pix_to_ht * np.complex128(c_pix[i]))
+        with mpmath.workprec(200):
+            w = complex(mpmath.exp(mpmath.mpc(arg.real, arg.imag)))
+        assert abs(w - cp[i]) <= 4e-16 * abs(w)
+        n, z, dz = brute.holo_escape(f, c_pix[i], 2, args.fields["max_iter"], 1e3)
+        same_n += int(n == off[3][0, i])
+        # pixels that escape within ~1.1 periods of the period-643 reference
+        # cycle are not chaotic: their escaped z_n agrees; those that shadow
+        # the cycle longer amplify binary64 rounding (the reference's as much)
+        if off[2][0, i] == 1 and n == off[3][0, i] and n <= 700:
+            assert abs(off[0][0, i] - z) <= 1e-6 * abs(z)
+            checked += 1
+    assert same_n >= len(sel) - 2 and checked >= len(sel) // 2
+    # dz/dc: the derivative paths are linear in their scale, so the field
+    # at dx exp(h_moy) is exp(h_moy) times the field at dx (the recurrence
+    # itself is pinned by brute force on the Cartesian frames)
+    unit = orc.OracleCtx(dict(args.fields, bla_on=0, deriv_mant=0., deriv_exp=0), args.arrays).cycles(cp)
+    esc = off[2][0] == 1
+    np.testing.assert_allclose(off[0][1, esc], unit[0][1, esc] * scale, rtol=1e-12)
