@@ -218,11 +218,21 @@ static int max_lds_bytes() {
             v = 0;
         cached = v;
     }
+    // FSG_LDS_BYTES caps the staging budget (to compare partial plans)
+    const char *cap = std::getenv("FSG_LDS_BYTES");
+    if (cap) {
+        const long c = std::strtol(cap, nullptr, 10);
+        if (c >= 0 && c < cached) return (int)c;
+    }
     return cached;
 }
 
-// Byte offsets of every table the loop reads, staged in LDS; bytes = 0 when
-// they do not fit in max_bytes.
+// Byte offsets of the tables staged in LDS (-1: read from global memory).
+// Entries 0 .. K-1 of each staged table (K = the iterations a pixel can
+// reach, min(L, div_iter + 1, order)), with the wrapped dZndz value at [K].
+// The tables are taken in the order the loop reads them most -- orbit,
+// Xrange slots, dZndz, dZndc (burning ship: its four paths), BLA radii and
+// coefficients -- as long as they fit; bytes = 0 when not even the orbit does.
 LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     LdsPlan p;
     std::memset(&p, 0xff, sizeof(p));               // every offset -1
@@ -231,20 +241,38 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     const int64_t bla_len = 2 * (int64_t)(t.L >> 3);
     const int64_t KB = t.bla_on ? std::min<int64_t>(bla_len, K / 4 + 2) : 0;
     int64_t off = 0;
-    auto take = [&](int64_t bytes) { int64_t o = off; off += (bytes + 15) & ~(int64_t)15; return (int32_t)o; };
+    const auto al = [](int64_t b) { return (b + 15) & ~(int64_t)15; };
+    // reserve a group of tables if it fits; returns false (nothing taken) otherwise
+    const auto group = [&](std::initializer_list<int64_t> sizes) {
+        int64_t total = 0;
+        for (int64_t b : sizes) total += al(b);
+        if (off + total > max_bytes || off + total >= INT32_MAX) return false;
+        return true;
+    };
+    const auto take = [&](int64_t bytes) { int64_t o = off; off += al(bytes); return (int32_t)o; };
+    if (!group({16 * K})) { p.K = (int32_t)K; p.KB = 0; p.bla_dim = bla_dim; p.bytes = 0; return p; }
     p.zn = take(16 * K);
-    if (t.dc) p.dc = take(16 * K);
-    if (t.dc_e) p.dc_e = take(4 * K);
-    if (t.dz) p.dz = take(16 * (K + 1));
-    if (t.dz_e) p.dz_e = take(4 * (K + 1));
-    for (int q = 0; q < 4; q++) {
-        if (t.bs[q]) p.bs[q] = take(8 * K);
-        if (t.bs_e[q]) p.bs_e[q] = take(4 * K);
+    if (t.xr_pos && group({4 * K})) p.xr_pos = take(4 * K);
+    if (t.dz && group({16 * (K + 1), t.dz_e ? 4 * (K + 1) : 0})) {
+        p.dz = take(16 * (K + 1));
+        if (t.dz_e) p.dz_e = take(4 * (K + 1));
     }
-    if (t.xr_pos) p.xr_pos = take(4 * K);
-    if (KB > 0) { p.r = take(8 * KB); p.M = take(8 * bla_dim * KB); }
+    if (t.dc && group({16 * K, t.dc_e ? 4 * K : 0})) {
+        p.dc = take(16 * K);
+        if (t.dc_e) p.dc_e = take(4 * K);
+    }
+    if (t.bs[0]) {
+        const int64_t eb = t.bs_e[0] ? 4 * K : 0;
+        if (group({8 * K, eb, 8 * K, eb, 8 * K, eb, 8 * K, eb})) {
+            for (int q = 0; q < 4; q++) {
+                if (t.bs[q]) p.bs[q] = take(8 * K);
+                if (t.bs_e[q]) p.bs_e[q] = take(4 * K);
+            }
+        }
+    }
+    if (KB > 0 && group({8 * KB, 8 * bla_dim * KB})) { p.r = take(8 * KB); p.M = take(8 * bla_dim * KB); }
     p.K = (int32_t)K; p.KB = (int32_t)KB; p.bla_dim = bla_dim;
-    p.bytes = (off <= max_bytes && off < INT32_MAX) ? (int32_t)off : 0;
+    p.bytes = (int32_t)off;
     return p;
 }
 

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(1024) k_tile_lds(const KArgs a) {
     t.xr_pos = stage(lds, p.xr_pos, a.t.xr_pos, K);
     t.r = stage(lds, p.r, a.t.r, KB);
     t.M = stage(lds, p.M, a.t.M, KB * p.bla_dim);
-    t.L = (int32_t)K;          // the wrapped dZndz value is staged at [K]
+    if (p.dz >= 0 && a.t.dz) t.L = (int32_t)K;   // the wrapped dZndz value is staged at [K]
     __syncthreads();
     tile_loop<Model>(al);
 }

@@ -182,12 +182,16 @@ def test_tables_global_and_lds_bitexact(case, monkeypatch):
     args, c_pix = frames.frame_inputs(f)
     args = with_fields(args, xr_mode=int(parts[-1]))
     ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
-    for mode in ("lds", "global"):
+    # all tables in LDS, in global memory, and partial plans (the orbit and
+    # whatever else fits a capped budget in LDS, the rest global)
+    for mode in ("lds", "global", "6000", "12000", "24000"):
         print(f"case tables {case} {mode}", flush=True)
+        monkeypatch.delenv("FSG_TABLES", raising=False)
+        monkeypatch.delenv("FSG_LDS_BYTES", raising=False)
         if mode == "global":
             monkeypatch.setenv("FSG_TABLES", "global")
-        else:
-            monkeypatch.delenv("FSG_TABLES", raising=False)
+        elif mode != "lds":
+            monkeypatch.setenv("FSG_LDS_BYTES", mode)
         got, _ = run_gpu(with_fields(args), c_pix)
         assert_same(got, ref)
 
