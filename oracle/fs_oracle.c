@@ -23,7 +23,12 @@
  *   - BLA build: merge == explicit composition of the halves; radii
  *     non-increasing under merge.
  *   - derivative paths: linear in the scale (dZndc(s) = s * dZndc(1)) and vs
- *     mpmath recurrences.
+ *     mpmath recurrences; the scale is dx, or dx * projection.scale
+ *     (scale_deriv_xr, perturbation.py:459-467) for the exponential map.
+ *
+ * Parity unpinned: the exponential map's projected pixel offsets are an
+ * input here (computed by the caller with numpy's complex exp); the
+ * reference's numba exp may differ from it in the last bit of an offset.
  */
 #include <stdint.h>
 #include <string.h>
