@@ -88,7 +88,21 @@ def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
         "kind": "port",
         "sample": f"every {stride}th pixel of the frame ({sample.size} px), {dt:.1f} s, "
                   "C restatement of the reference numba loop, OpenMP dynamic schedule",
-    }
+    }, stride, (Z, U, sr, si)
+
+
+def glitch_report(res, fields):
+    """The glitch criterion on the binary64 results: pixels left unfinished
+    (stop reason -1) and non-finite z_n or dz/dc."""
+    Z, _, sr, _ = res
+    if fields["kind"] == 2:
+        rows = [0, 1] + ([2, 3, 4, 5] if fields["calc_dzndc"] else [])
+    else:
+        rows = [0] + ([1 + int(bool(fields["interior_detect"]))] if fields["calc_dzndc"] else [])
+    nonfinite = np.zeros(Z.shape[1], dtype=bool)
+    for r in rows:
+        nonfinite |= ~np.isfinite(Z[r])
+    return {"unfinished": int(np.count_nonzero(sr < 0)), "non_finite": int(np.count_nonzero(nonfinite))}
 
 
 def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
@@ -333,16 +347,18 @@ def main():
         cia = f._calc_data["bench"]["cycle_indep_args"]
         prep_s = time.perf_counter() - t0
         leg64 = Leg(f, cia, 0, torch, dev).run(steps, warmup)
+        res64 = leg64.host_results()
         entry = {"model": w.model + (f" d={w.degree}" if w.model == "MN" else ""), "steps": steps,
                  "width": w.width, "pixels": leg64.n, "host_prep_s": round(prep_s, 2),
                  "ref_len": cia.fields["ref_len"], "ref_order": f.FP_params["order"],
                  "ref_div_iter": f.FP_params["div_iter"], "xr_mode": cia.fields["xr_mode"],
                  "fp64": leg64.summary()}
+        entry["fp64"]["glitched"] = glitch_report(res64, cia.fields)
         if not args.no_fp32x:
             leg32 = Leg(f, cia, 1, torch, dev).run(steps, warmup)
             entry["fp32x"] = leg32.summary()
             entry["fp32x"]["speedup_vs_fp64"] = leg32.value / leg64.value
-            entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), leg64.host_results())
+            entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), res64)
             del leg32
         frames[wname] = entry
         if main_leg is None:
@@ -395,7 +411,15 @@ def main():
     if args.movie > 0:
         out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=960, ny=540)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
+        base, stride, ref = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
+        # the same sample checked against the GPU's results of the timed frame
+        got = [np.ascontiguousarray(a[:, ::stride]) for a in leg.host_results()]
+        same = (np.array_equal(got[0].view(np.uint64), np.ascontiguousarray(ref[0]).view(np.uint64))
+                and all(np.array_equal(g, r) for g, r in zip(got[1:], ref[1:])))
+        far = int(np.count_nonzero(np.abs(got[3].astype(np.int64) - ref[3]) > 1))
+        base["oracle_check"] = {"pixels": int(ref[3].size), "bit_exact": bool(same),
+                                "escape_count_beyond_pm1": far}
+        out["cpu_baseline"] = base
     print(json.dumps(out), flush=True)
 
 
