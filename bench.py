@@ -230,9 +230,10 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
 class Leg:
     """One workload x precision on the GPU: device buffers + timed steps."""
 
-    def __init__(self, f, cia, precision, torch, dev):
+    def __init__(self, f, cia, precision, torch, dev, grid=True):
         from fractalshades_amd._native import work_bytes
         self.f = f
+        self.grid_w = f.nx if grid else 0     # 8x8 pixel blocks per wave, or index order
         self.cia = cia if precision == cia.fields["precision"] else cia.with_fields(precision=precision)
         self.ctx = self.cia.ctx
         self.torch = torch
@@ -255,7 +256,7 @@ class Leg:
     def step(self):
         self.ctx.tile_device(self.d_pix.data_ptr(), self.n, self.d_Z.data_ptr(), self.d_U.data_ptr(),
                              self.d_sr.data_ptr(), self.d_si.data_ptr(), self.d_work.data_ptr(),
-                             self.stream.cuda_stream)
+                             self.stream.cuda_stream, grid_w=self.grid_w)
 
     def run(self, steps, warmup):
         from fractalshades_amd._native import work_stats
@@ -312,6 +313,8 @@ def main():
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
     ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
     ap.add_argument("--movie", type=int, default=64, help="frames of the zoom-sequence leg (0: skip)")
+    ap.add_argument("--order", choices=("grid", "linear"), default="grid",
+                    help="pixel order of the kernel: 8x8 blocks of the frame, or index order")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 256 MiB device copy (PMC counter calibration)")
     args = ap.parse_args()
@@ -346,7 +349,8 @@ def main():
         f = configs.build(w)
         cia = f._calc_data["bench"]["cycle_indep_args"]
         prep_s = time.perf_counter() - t0
-        leg64 = Leg(f, cia, 0, torch, dev).run(steps, warmup)
+        grid = args.order == "grid"
+        leg64 = Leg(f, cia, 0, torch, dev, grid).run(steps, warmup)
         res64 = leg64.host_results()
         entry = {"model": w.model + (f" d={w.degree}" if w.model == "MN" else ""), "steps": steps,
                  "width": w.width, "pixels": leg64.n, "host_prep_s": round(prep_s, 2),
@@ -355,7 +359,7 @@ def main():
                  "fp64": leg64.summary()}
         entry["fp64"]["glitched"] = glitch_report(res64, cia.fields)
         if not args.no_fp32x:
-            leg32 = Leg(f, cia, 1, torch, dev).run(steps, warmup)
+            leg32 = Leg(f, cia, 1, torch, dev, grid).run(steps, warmup)
             entry["fp32x"] = leg32.summary()
             entry["fp32x"]["speedup_vs_fp64"] = leg32.value / leg64.value
             entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), res64)
@@ -386,6 +390,7 @@ def main():
                         f"max_iter {w.max_iter}, interior_detect {w.interior_detect}, "
                         f"BLA_eps 1e-6, FP64 drop-in (Xrange deltas)",
             "pixels": leg.n, "pixel_iters_per_step": leg.pix_iters,
+            "pixel_order": "8x8 blocks" if args.order == "grid" else "index",
             "frames": frames,
         },
         "roofline": {

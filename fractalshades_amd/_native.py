@@ -19,7 +19,7 @@ FSG_INTERRUPTED = 1
 # entry points declared in include/fsgpu.h
 EXPORTS = (
     "fsg_abi_version", "fsg_ctx_create", "fsg_ctx_create_shared", "fsg_ctx_destroy", "fsg_tile",
-    "fsg_tile_device", "fsg_work_bytes", "fsg_work_stats",
+    "fsg_tile_device", "fsg_tile_grid", "fsg_tile_grid_device", "fsg_work_bytes", "fsg_work_stats",
     "fsg_ctx_read_table", "fsg_ctx_bla_shape", "fsg_last_error",
     "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
     "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
@@ -166,6 +166,8 @@ def load(path=None):
             "fsg_ctx_destroy": ([vp], ctypes.c_int),
             "fsg_tile": ([vp, vp, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "fsg_tile_device": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "fsg_tile_grid": ([vp, vp, i64, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
+            "fsg_tile_grid_device": ([vp, vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "fsg_work_bytes": ([], ctypes.c_size_t),
             "fsg_work_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "fsg_ctx_read_table": ([vp, ctypes.c_int, vp, ctypes.c_size_t], ctypes.c_int),
@@ -260,8 +262,9 @@ class GpuContext:
         except Exception:
             pass
 
-    def tile(self, c_pix, Z, U, stop_reason, stop_iter, interrupted=None):
-        """Run one tile in place; returns (rc, stats dict)."""
+    def tile(self, c_pix, Z, U, stop_reason, stop_iter, interrupted=None, grid_w=0):
+        """Run one tile in place; returns (rc, stats dict).  grid_w > 0: c_pix
+        holds rows of grid_w pixels, computed in 8x8 blocks (same results)."""
         for name, arr in (("c_pix", c_pix), ("Z", Z), ("U", U),
                           ("stop_reason", stop_reason), ("stop_iter", stop_iter)):
             if not arr.flags.c_contiguous:
@@ -278,15 +281,15 @@ class GpuContext:
         intr = None
         if interrupted is not None:
             intr = interrupted.ctypes.data
-        rc = check(load().fsg_tile(
-            self._h, _ptr(c_pix), n, _ptr(Z), _ptr(U), _ptr(stop_reason), _ptr(stop_iter),
-            intr, ctypes.byref(st)))
+        rc = check(load().fsg_tile_grid(
+            self._h, _ptr(c_pix), n, int(grid_w), _ptr(Z), _ptr(U), _ptr(stop_reason),
+            _ptr(stop_iter), intr, ctypes.byref(st)))
         return rc, st.as_dict()
 
-    def tile_device(self, c_pix_ptr, n, Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr):
+    def tile_device(self, c_pix_ptr, n, Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr, grid_w=0):
         """Enqueue one tile on device buffers (raw pointers); asynchronous."""
-        return check(load().fsg_tile_device(
-            self._h, c_pix_ptr, n, Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr))
+        return check(load().fsg_tile_grid_device(
+            self._h, c_pix_ptr, n, int(grid_w), Z_ptr, U_ptr, sr_ptr, si_ptr, work_ptr, stream_ptr))
 
     def bla_shape(self):
         bl = ctypes.c_int64()

@@ -190,7 +190,10 @@ class Fractal:
         (c_pix, Z, U, stop_reason, stop_iter), subset = self.get_cycling_dep_args(
             calc_name, chunk_slice, jitter, supersampling)
         cycle_indep_args = self._calc_data[calc_name]["cycle_indep_args"]
-        ret = self.numba_cycle_call((c_pix, Z, U, stop_reason, stop_iter), cycle_indep_args)
+        ix, ixx, _, _ = chunk_slice
+        grid_w = 0 if subset is not None else (ixx - ix) * (supersampling or 1)
+        ret = self.numba_cycle_call((c_pix, Z, U, stop_reason, stop_iter), cycle_indep_args,
+                                    grid_w=grid_w)
         return ret, (c_pix, Z, U, stop_reason, stop_iter)
 
     def calc_raw(self, calc_name, jitter=False, supersampling=None):
@@ -205,10 +208,13 @@ class Fractal:
         state = self._calc_data[calc_name]["state"]
         c_pix = np.ravel(self.chunk_pixel_pos((0, self.nx, 0, self.ny), jitter, supersampling))
         subset = getattr(state, "subset", None)
+        grid_w = self.nx * (supersampling or 1)
         if subset is not None:
             c_pix = c_pix[np.ravel(np.asarray(subset, dtype=bool))]
+            grid_w = 0
         args = self._result_arrays(state, c_pix)
-        ret = self.numba_cycle_call(args, self._calc_data[calc_name]["cycle_indep_args"])
+        ret = self.numba_cycle_call(args, self._calc_data[calc_name]["cycle_indep_args"],
+                                    grid_w=grid_w)
         c_pix, Z, U, stop_reason, stop_iter = args
         return {"ret": ret, "c_pix": c_pix, "Z": Z, "U": U, "stop_reason": stop_reason,
                 "stop_iter": stop_iter, "codes": state.codes,

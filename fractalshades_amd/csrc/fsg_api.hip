@@ -350,11 +350,12 @@ size_t z_bytes(const fsg_ctx *c, int64_t n) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8_t *sr, int32_t *si,
-           unsigned long long *work, hipStream_t stream, bool timed, int64_t ld = -1, int max_blocks = 0) {
+           unsigned long long *work, hipStream_t stream, bool timed, int64_t ld = -1, int max_blocks = 0,
+           int64_t grid_w = 0) {
     KArgs a;
     fill_args(c, a);
     a.pix = pix; a.npts = n; a.ld = ld < 0 ? n : ld; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si;
-    a.work = work; a.max_blocks = max_blocks;
+    a.work = work; a.max_blocks = max_blocks; a.grid_w = grid_w;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, work, WORK_WORDS);
     FSG_HIP(hipGetLastError());
     if (n == 0) return 0;
@@ -584,9 +585,16 @@ int fsg_ctx_read_table(fsg_ctx *c, int which, void *out, size_t bytes) {
 
 int fsg_tile(fsg_ctx *c, const double *c_pix, int64_t n, double *Z, int32_t *U, int8_t *stop_reason,
              int32_t *stop_iter, const volatile uint8_t *interrupted, fsg_stats *stats) {
+    return fsg_tile_grid(c, c_pix, n, 0, Z, U, stop_reason, stop_iter, interrupted, stats);
+}
+
+int fsg_tile_grid(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, double *Z, int32_t *U,
+                  int8_t *stop_reason, int32_t *stop_iter, const volatile uint8_t *interrupted,
+                  fsg_stats *stats) {
     if (!c) return fail(FSG_E_INVALID, "null context");
     if (n < 0 || (n > 0 && (!c_pix || !Z || !U || !stop_reason || !stop_iter)))
         return fail(FSG_E_INVALID, "bad tile buffers");
+    if (grid_w < 0) return fail(FSG_E_INVALID, "negative grid width");
     if (interrupted && *interrupted) return FSG_INTERRUPTED;
     std::lock_guard<std::mutex> lk(c->mu);
     const size_t pb = (size_t)n * 16, zb = z_bytes(c, n), ub = (size_t)n * 4, rb = (size_t)n;
@@ -601,7 +609,7 @@ int fsg_tile(fsg_ctx *c, const double *c_pix, int64_t n, double *Z, int32_t *U, 
     if (n > 0) FSG_HIP(hipMemcpyAsync(b, c_pix, pb, hipMemcpyHostToDevice, c->stream));
     int rc = launch(c, reinterpret_cast<double *>(b), n, reinterpret_cast<double *>(b + off_z),
                     reinterpret_cast<int32_t *>(b + off_u), reinterpret_cast<int8_t *>(b + off_r),
-                    reinterpret_cast<int32_t *>(b + off_i), c->work, c->stream, true);
+                    reinterpret_cast<int32_t *>(b + off_i), c->work, c->stream, true, -1, 0, grid_w);
     if (rc) return rc;
     if (n > 0) {
         FSG_HIP(hipMemcpyAsync(Z, b + off_z, zb, hipMemcpyDeviceToHost, c->stream));
@@ -623,12 +631,20 @@ int fsg_tile(fsg_ctx *c, const double *c_pix, int64_t n, double *Z, int32_t *U, 
 
 int fsg_tile_device(fsg_ctx *c, const double *d_c_pix, int64_t n, double *d_Z, int32_t *d_U,
                     int8_t *d_stop_reason, int32_t *d_stop_iter, void *d_work, void *stream) {
+    return fsg_tile_grid_device(c, d_c_pix, n, 0, d_Z, d_U, d_stop_reason, d_stop_iter, d_work, stream);
+}
+
+int fsg_tile_grid_device(fsg_ctx *c, const double *d_c_pix, int64_t n, int64_t grid_w, double *d_Z,
+                         int32_t *d_U, int8_t *d_stop_reason, int32_t *d_stop_iter, void *d_work,
+                         void *stream) {
     if (!c) return fail(FSG_E_INVALID, "null context");
     if (!d_work) return fail(FSG_E_INVALID, "null work buffer");
     if (n < 0 || (n > 0 && (!d_c_pix || !d_Z || !d_U || !d_stop_reason || !d_stop_iter)))
         return fail(FSG_E_INVALID, "bad tile buffers");
+    if (grid_w < 0) return fail(FSG_E_INVALID, "negative grid width");
     return launch(c, d_c_pix, n, d_Z, d_U, d_stop_reason, d_stop_iter,
-                  static_cast<unsigned long long *>(d_work), static_cast<hipStream_t>(stream), false);
+                  static_cast<unsigned long long *>(d_work), static_cast<hipStream_t>(stream), false,
+                  -1, 0, grid_w);
 }
 
 int fsg_work_stats(const void *d_work, fsg_stats *stats) {

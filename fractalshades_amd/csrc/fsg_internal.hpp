@@ -61,6 +61,9 @@ struct KArgs {
     // stop_reason int8
     double *Z; int32_t *U; int8_t *stop_reason; int32_t *stop_iter;
     int64_t ld;
+    // > 0: the pixels are rows of grid_w (the last row may be short), taken
+    // in 8x8 blocks (grid_slot); 0: taken in index order
+    int64_t grid_w;
     int32_t nz; int32_t max_blocks;   // grid cap: 0 fill the GPU, > 0 blocks, -k 1/k of the GPU
     int32_t calc_orbit, backshift;    // orbit point at exit - backshift in the last field(s)
     LdsPlan lds;                      // lds.bytes == 0: tables read from global memory

@@ -349,14 +349,15 @@ class PerturbationFractal(Fractal):
         return CycleIndepArgs(fields, arrays, self._interrupted, projection=proj)
 
     @staticmethod
-    def numba_cycle_call(cycle_dep_args, cycle_indep_args):
+    def numba_cycle_call(cycle_dep_args, cycle_indep_args, grid_w=0):
         """One tile on the GPU; returns 0, or 1 when interrupted
-        (perturbation.py:414-428)."""
+        (perturbation.py:414-428).  grid_w > 0: c_pix holds rows of grid_w
+        pixels, which the kernel takes in 8x8 blocks (same results)."""
         c_pix, Z, U, stop_reason, stop_iter = cycle_dep_args
         args = cycle_indep_args
         proj = args.projection
         c_in = c_pix if proj is None else np.ascontiguousarray(proj.f(c_pix))
-        rc, stats = args.ctx.tile(c_in, Z, U, stop_reason, stop_iter, args.interrupted)
+        rc, stats = args.ctx.tile(c_in, Z, U, stop_reason, stop_iter, args.interrupted, grid_w=grid_w)
         args.last_stats = stats
         if proj is not None and proj.dzndc_modifier is not None:
             # perturbation.py:1387-1388, 1773-1777

@@ -169,6 +169,23 @@ int fsg_tile_device(fsg_ctx *ctx, const double *d_c_pix, int64_t n, double *d_Z,
                     int32_t *d_U, int8_t *d_stop_reason, int32_t *d_stop_iter,
                     void *d_work, void *stream);
 
+/*
+ * fsg_tile and fsg_tile_device over a pixel grid: c_pix holds rows of grid_w
+ * pixels (the layout of core.py:1767-1830 for a tile of width grid_w; the
+ * last row may be short) and the kernel takes them in 8x8 blocks, so that
+ * the 64 lanes of a wave iterate neighbouring pixels.  Results are the same
+ * as fsg_tile's, at the same indices; only the order in which pixels are
+ * computed changes.  grid_w = 0 is fsg_tile / fsg_tile_device.  Replaces the
+ * same reference entry point as fsg_tile (its caller knows the tile's
+ * width from chunk_slice, core.py:2044-2075).
+ */
+int fsg_tile_grid(fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w, double *Z,
+                  int32_t *U, int8_t *stop_reason, int32_t *stop_iter,
+                  const volatile uint8_t *interrupted, fsg_stats *stats);
+int fsg_tile_grid_device(fsg_ctx *ctx, const double *d_c_pix, int64_t n, int64_t grid_w,
+                         double *d_Z, int32_t *d_U, int8_t *d_stop_reason, int32_t *d_stop_iter,
+                         void *d_work, void *stream);
+
 /* Size of the `work` buffer of fsg_tile_device. No reference counterpart. */
 size_t fsg_work_bytes(void);
 

@@ -5,7 +5,7 @@ import numpy as np
 from fractalshades_amd._native import n_z_fields
 
 
-def run_gpu(args, c_pix):
+def run_gpu(args, c_pix, grid_w=0):
     kind = args.fields["kind"]
     n = c_pix.shape[0]
     nz = n_z_fields(args.fields)
@@ -13,7 +13,7 @@ def run_gpu(args, c_pix):
     U = np.zeros((1, n), dtype=np.int32)
     sr = -np.ones((1, n), dtype=np.int8)
     si = np.zeros((1, n), dtype=np.int32)
-    rc, stats = args.ctx.tile(np.ascontiguousarray(c_pix), Z, U, sr, si)
+    rc, stats = args.ctx.tile(np.ascontiguousarray(c_pix), Z, U, sr, si, grid_w=grid_w)
     assert rc == 0
     return (Z, U, sr, si), stats
 

@@ -84,6 +84,39 @@ def test_workload_subsample_bitexact(name):
     sub = tuple(np.ascontiguousarray(a[..., sel]) for a in got)
     assert_same(sub, ref)
     assert np.all(got[2] >= 0)
+    # the whole frame again in 8x8 pixel blocks (the product path's order):
+    # the same results at the same indices
+    print(f"case {name} grid_w={f.nx}", flush=True)
+    got_grid, stats_grid = run_gpu(args, c_pix, grid_w=f.nx)
+    assert_same(got_grid, got)
+    assert stats_grid["plain_steps"] == stats["plain_steps"]
+
+
+@pytest.mark.parametrize("frame", ["M2_E20-1", "Mn3-0", "BS_ex14-Celtic-1"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grid_order_bitexact(frame, precision):
+    """Pixels taken in 8x8 blocks of rows of grid_w: the oracle's results at
+    the same indices, for row widths that divide the frame, that leave a
+    ragged right strip and a short last row, narrower than a block, of one
+    pixel, and wider than the whole input."""
+    parts = frame.split("-")
+    if parts[0] == "Mn3":
+        f = frames.mn_frame(3, nx=45)
+    elif parts[0].startswith("M2"):
+        f = frames.make(parts[0], nx=50)
+    else:
+        f = frames.make(parts[0], nx=44, model_kw={"flavor": parts[1]}, max_iter=6000)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=int(parts[-1]), precision=precision)
+    if precision == 0:
+        ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    else:
+        ref, _ = run_gpu(args, c_pix)      # binary32 path: index order
+    n = c_pix.size
+    for w in (f.nx, 37, 8, 5, 1, n + 3):
+        print(f"case grid {frame} precision={precision} grid_w={w} n={n}", flush=True)
+        got, _ = run_gpu(args, c_pix, grid_w=w)
+        assert_same(got, ref)
 
 
 @pytest.mark.parametrize("d", [3, 4, 6])
