@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--deep", default="C2_2000", help="second frame (1e-2000); '' to skip")
-    ap.add_argument("--extra", default="C3_d3,C3_d4,C3_d6",
+    ap.add_argument("--extra", default="C3_d3,C3_d4,C3_d6,C3_bs",
                     help="further workloads reported under config.frames (fewer steps); '' to skip")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
@@ -362,7 +362,9 @@ def main():
             leg32 = Leg(f, cia, 1, torch, dev, grid).run(steps, warmup)
             entry["fp32x"] = leg32.summary()
             entry["fp32x"]["speedup_vs_fp64"] = leg32.value / leg64.value
-            entry["fp32x"]["error_vs_fp64"] = error_report(leg32.host_results(), res64)
+            entry["fp32x"]["error_vs_fp64"] = error_report(
+                leg32.host_results(), res64, holomorphic=cia.fields["kind"] != 2,
+                d=cia.fields["degree"] if cia.fields["kind"] == 1 else 2)
             del leg32
         frames[wname] = entry
         if main_leg is None:

@@ -153,6 +153,96 @@ def descend_to_boundary(c0, target_log2, prec, nmax=20000, degree=2, max_steps=1
     return None
 
 
+def _bs_partials(flavor, x, y, p):
+    """(X, Y, dX/dx, dX/dy, dY/dx, dY/dy) of the burning-ship fold-and-square
+    (burning_ship.py:63-76 flavours 1..5; x' = X + a, y' = Y - b), in fixed
+    point with p fractional bits."""
+    sg = lambda v: -1 if v < 0 else 1
+    if flavor == 3:                                   # x^2 - y|y|, 2xy
+        X = (x * x - y * abs(y)) >> p
+        Y = (2 * x * y) >> p
+        return X, Y, 2 * x, -2 * abs(y), 2 * y, 2 * x
+    X = (x * x - y * y) >> p
+    dXx, dXy = 2 * x, -2 * y
+    if flavor in (4, 5):                              # |x^2 - y^2|
+        s = sg(X)
+        X, dXx, dXy = abs(X), s * dXx, s * dXy
+    if flavor == 2:                                   # 2 x |y|
+        Y = (2 * x * abs(y)) >> p
+        dYx, dYy = 2 * abs(y), 2 * x * sg(y)
+    else:
+        Y = (2 * x * y) >> p
+        dYx, dYy = 2 * y, 2 * x
+        if flavor in (1, 5):                          # 2 |x y|
+            s = sg(Y)
+            Y, dYx, dYy = abs(Y), s * dYx, s * dYy
+    return X, Y, dXx, dXy, dYx, dYy
+
+
+def bs_escape_jacobian(a, b, p, nmax, flavor):
+    """Orbit of c = (a, b) (fixed point, p fractional bits) up to the first
+    n with |z_n|^2 > 1e10, with J = d(x_n, y_n)/d(a, b): returns
+    (n, x_n, y_n, Jxa, Jxb, Jya, Jyb) in the same fixed point, or None if
+    the orbit stays bounded for nmax steps."""
+    one = 1 << p
+    R2 = (10 ** 10) << (2 * p)
+    x = y = 0
+    jxa = jxb = jya = jyb = 0
+    for n in range(1, nmax + 1):
+        X, Y, dXx, dXy, dYx, dYy = _bs_partials(flavor, x, y, p)
+        jxa, jxb, jya, jyb = (((dXx * jxa + dXy * jya) >> p) + one, (dXx * jxb + dXy * jyb) >> p,
+                              (dYx * jxa + dYy * jya) >> p, ((dYx * jxb + dYy * jyb) >> p) - one)
+        x, y = X + a, Y - b
+        if x * x + y * y > R2:
+            return n, x, y, jxa, jxb, jya, jyb
+    return None
+
+
+def descend_to_boundary_bs(c0, target_log2, prec, nmax=20000, flavor=1, max_steps=100000):
+    """descend_to_boundary for the burning-ship family: with the 2x2
+    Jacobian J = d(x_n, y_n)/d(a, b), G = ln|z_n| / 2^n has the gradient
+    J^T z_n / (|z_n|^2 2^n), the distance estimate is
+    d = |z|^2 ln|z| / |J^T z| and each step moves c by
+    -(ln|z| / 2) |z|^2 J^T z / |J^T z|^2 (half the Newton step on G)."""
+    cx = int(mpmath.ldexp(mpmath.mpf(c0.real), prec))
+    cy = int(mpmath.ldexp(mpmath.mpf(c0.imag), prec))
+    d_log2 = 0.
+    for _ in range(max_steps):
+        p = min(prec, max(64, int(-d_log2) + 64))
+        sh = prec - p
+        one = 1 << p
+        esc = bs_escape_jacobian(cx >> sh, cy >> sh, p, nmax, flavor)
+        if esc is None:
+            return None
+        _, x, y, jxa, jxb, jya, jyb = esc
+        xf, yf = x / one, y / one
+        # g = J^T z (units 2^-p of J's fixed point, times z's float)
+        gms = [_fmant(v) for v in (jxa, jya, jxb, jyb)]
+        e = max(ge for _, ge in gms)
+        jf = [math.ldexp(m, ge - e) for m, ge in gms]            # J = jf * 2^(e - p)
+        ga = jf[0] * xf + jf[1] * yf
+        gb = jf[2] * xf + jf[3] * yf
+        ng = math.hypot(ga, gb)
+        az2 = xf * xf + yf * yf
+        lz = 0.5 * math.log(az2)
+        if ng == 0.:
+            return None
+        d_log2 = math.log2(az2 * lz / ng) - (e - p)
+        if d_log2 < target_log2:
+            return mpmath.mpc(mpmath.ldexp(cx, -prec), mpmath.ldexp(cy, -prec)), d_log2
+        k = -0.5 * lz * az2 / (ng * ng)                        # step = k g 2^(p - e)
+        shift = prec + p - e
+
+        def to_fixed(v):
+            m, ex = math.frexp(v)
+            q = int(m * (1 << 53))
+            t = ex - 53 + shift
+            return q << t if t >= 0 else q >> (-t)
+        cx += to_fixed(k * ga)
+        cy += to_fixed(k * gb)
+    return None
+
+
 def seeded_center(w, max_tries=40, nmax=None):
     """(x, y, nucleus, order) for workload w (see module doc)."""
     rng = np.random.default_rng(w.seed)
@@ -168,8 +258,12 @@ def seeded_center(w, max_tries=40, nmax=None):
         for _ in range(max_tries):
             cx = BOX[0] + (BOX[1] - BOX[0]) * rng.random()
             cy = BOX[2] + (BOX[3] - BOX[2]) * rng.random()
-            res = descend_to_boundary(complex(cx, cy), float(mpmath.log(width, 2)) - 10,
-                                      bits, nmax, w.degree) if w.model != "BS" else None
+            if w.model == "BS":
+                res = descend_to_boundary_bs(complex(cx, cy), float(mpmath.log(width, 2)) - 10,
+                                             bits, nmax, flavor)
+            else:
+                res = descend_to_boundary(complex(cx, cy), float(mpmath.log(width, 2)) - 10,
+                                          bits, nmax, w.degree)
             if res is None:
                 continue
             cb, _ = res

@@ -311,6 +311,13 @@ __device__ __forceinline__ xsr operator-(xsr a, xsr b) { return a + (-b); }
 __device__ __forceinline__ xsr operator*(xsr a, xsr b) { return xsr_n(a.m * b.m, a.e + b.e); }
 __device__ __forceinline__ xsr operator*(double s, xsr a) { return xsr_n((float)s * a.m, a.e); }
 __device__ __forceinline__ xsr mul_s(double s, xsr a) { return xsr_n((float)s * a.m, a.e); }
+// s * a for a binary64 s of any magnitude: the BLA coefficients of deep
+// frames leave the binary32 range (1e-800: |M| up to ~1e300), so s is split
+// into a binary32 mantissa and an exponent first
+__device__ __forceinline__ xsr mul_w(double s, xsr a) {
+    const int32_t k = __builtin_amdgcn_frexp_exp(s) - 1;
+    return xsr_n((float)ldexp(s, -k) * a.m, a.e + k);
+}
 __device__ __forceinline__ bool xsr_le(xsr a, xsr b) {   // a <= b
     const int32_t E = max(a.e, b.e);
     return ldexpf(a.m, max(a.e - E, -200)) <= ldexpf(b.m, max(b.e - E, -200));
@@ -362,15 +369,15 @@ struct BsF32 {
             {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
-                const xsr nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
-                const xsr ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
+                const xsr nx = mul_w(M[0], s.x) + mul_w(M[1], s.y) + mul_w(M[4], s.a) + mul_w(M[5], s.b);
+                const xsr ny = mul_w(M[2], s.x) + mul_w(M[3], s.y) + mul_w(M[6], s.a) + mul_w(M[7], s.b);
                 s.x = nx; s.y = ny;
                 s.xs = xsr_to_d(s.x); s.ys = xsr_to_d(s.y);
                 if (a.calc_dc) {
-                    const xsr p0 = M[0] * s.P[0] + M[1] * s.P[2];
-                    const xsr p1 = M[0] * s.P[1] + M[1] * s.P[3];
-                    const xsr p2 = M[2] * s.P[0] + M[3] * s.P[2];
-                    const xsr p3 = M[2] * s.P[1] + M[3] * s.P[3];
+                    const xsr p0 = mul_w(M[0], s.P[0]) + mul_w(M[1], s.P[2]);
+                    const xsr p1 = mul_w(M[0], s.P[1]) + mul_w(M[1], s.P[3]);
+                    const xsr p2 = mul_w(M[2], s.P[0]) + mul_w(M[3], s.P[2]);
+                    const xsr p3 = mul_w(M[2], s.P[1]) + mul_w(M[3], s.P[3]);
                     s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
                 }
                 k.bla++; k.bla_it += st;

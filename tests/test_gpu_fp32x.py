@@ -51,6 +51,25 @@ def test_fp32x_workloads(name):
     assert rep["escape_count_mismatch_frac"] <= 1e-4
 
 
+@pytest.mark.parametrize("name", ["C3_d4", "C3_bs"])
+def test_fp32x_deep_workloads(name):
+    """The 1e-800 frames (reduced to 480x270 pixels): BLA coefficients and
+    derivatives far outside the binary32 range."""
+    try:
+        configs.center_for(name, compute=False)
+    except KeyError:
+        pytest.skip(f"no cached seeded centre for {name}")
+    w = configs.WORKLOADS[name]
+    f = configs.build(w, nx=480, ny=270)
+    args = f._calc_data["bench"]["cycle_indep_args"]
+    c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
+    print(f"case fp32x {name} npts={c_pix.size}", flush=True)
+    rep = compare(args, c_pix, d=w.degree if w.model == "MN" else 2)
+    assert rep["unfinished"] == 0 and rep["non_finite_zn"] == 0
+    assert rep["escape_count_mismatch_frac"] < 0.01
+    assert rep["stop_reason_mismatch"] == 0
+
+
 @pytest.mark.parametrize("name,flavor", [("BS_ex14", "Burning ship"), ("PBS_ex18", "Perpendicular burning ship")])
 @pytest.mark.parametrize("xr", [0, 1])
 def test_fp32x_burning_ship_deep(name, flavor, xr):

@@ -66,11 +66,11 @@ def _cached(name):
         return False
 
 
-@pytest.mark.parametrize("name", ["C1", "C2", "C2_2000", "C3_d3", "C3_d4", "C3_d6"])
+@pytest.mark.parametrize("name", ["C1", "C2", "C2_2000", "C3_d3", "C3_d4", "C3_d6", "C3_bs"])
 def test_workload_subsample_bitexact(name):
-    # full-size seeded benchmark frame on the GPU (C3: z^d + c at 1e-800,
-    # 1920x1080, max_iter 1e6), checked on a strided subsample against the
-    # oracle
+    # full-size seeded benchmark frame on the GPU (C3: z^d + c and the
+    # burning ship at 1e-800, 1920x1080, max_iter 1e6), checked on a strided
+    # subsample against the oracle
     if not _cached(name):
         pytest.skip(f"no cached seeded centre for {name}")
     w = configs.WORKLOADS[name]
