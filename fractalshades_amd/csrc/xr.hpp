@@ -85,23 +85,29 @@ FSG_HD xf norm_f(double m, int32_t e) {
     double nm = frexp1(m, ne);
     return mkf(nm, e + ne);
 }
-// co-exponent of two real mantissas
+// co-exponent of two real mantissas: the mantissa of the smaller exponent
+// is shifted to the larger one, unless either is zero.  Both candidates are
+// formed and selected: a store through "whichever of m0 / m1 shifts" makes
+// the compiler place both in private (scratch) memory.
 FSG_HD void coexp(double &m0, int32_t e0, double &m1, int32_t e1, int32_t &e) {
-    int32_t d = e0 - e1;
-    if (m0 == 0.) e = e1;
-    else if (m1 == 0.) e = e0;
-    else if (e1 > e0) { m0 = exp2_shift(m0, d); e = e1; }
-    else if (e0 > e1) { m1 = exp2_shift(m1, -d); e = e0; }
-    else e = e0;
+    const int32_t d = e0 - e1;
+    const bool z0 = m0 == 0., z1 = m1 == 0.;
+    const double s0 = exp2_shift(m0, d < 0 ? d : 0), s1 = exp2_shift(m1, d > 0 ? -d : 0);
+    m0 = (!z0 && !z1 && e1 > e0) ? s0 : m0;
+    m1 = (!z0 && !z1 && e0 > e1) ? s1 : m1;
+    e = z0 ? e1 : (z1 ? e0 : (e0 > e1 ? e0 : e1));
 }
 // co-exponent of two complex mantissas
 FSG_HD void coexp(double &r0, double &i0, int32_t e0, double &r1, double &i1, int32_t e1, int32_t &e) {
-    int32_t d = e0 - e1;
-    if (r0 == 0. && i0 == 0.) e = e1;
-    else if (r1 == 0. && i1 == 0.) e = e0;
-    else if (e1 > e0) { r0 = exp2_shift(r0, d); i0 = exp2_shift(i0, d); e = e1; }
-    else if (e0 > e1) { r1 = exp2_shift(r1, -d); i1 = exp2_shift(i1, -d); e = e0; }
-    else e = e0;
+    const int32_t d = e0 - e1;
+    const bool z0 = r0 == 0. && i0 == 0., z1 = r1 == 0. && i1 == 0.;
+    const int32_t d0 = d < 0 ? d : 0, d1 = d > 0 ? -d : 0;
+    const double sr0 = exp2_shift(r0, d0), si0 = exp2_shift(i0, d0);
+    const double sr1 = exp2_shift(r1, d1), si1 = exp2_shift(i1, d1);
+    const bool sh0 = !z0 && !z1 && e1 > e0, sh1 = !z0 && !z1 && e0 > e1;
+    r0 = sh0 ? sr0 : r0; i0 = sh0 ? si0 : i0;
+    r1 = sh1 ? sr1 : r1; i1 = sh1 ? si1 : i1;
+    e = z0 ? e1 : (z1 ? e0 : (e0 > e1 ? e0 : e1));
 }
 FSG_HD xc norm_c(double re, double im, int32_t e) {
     xf r = norm_f(re, e), i = norm_f(im, e);
