@@ -414,16 +414,18 @@ struct BsXr {
             {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
-                xlf nx = M[0] * s.x + M[1] * s.y + M[4] * s.a + M[5] * s.b;
-                xlf ny = M[2] * s.x + M[3] * s.y + M[6] * s.a + M[7] * s.b;
-                s.x = nx; s.y = ny;
+                const xzf x = as_zf(s.x), y = as_zf(s.y);
+                const xzf ca = as_zf(s.a), cb = as_zf(s.b);
+                const xzf nx = mul_w(M[0], x) + mul_w(M[1], y) + mul_w(M[4], ca) + mul_w(M[5], cb);
+                const xzf ny = mul_w(M[2], x) + mul_w(M[3], y) + mul_w(M[6], ca) + mul_w(M[7], cb);
+                s.x = xlf_of(nx); s.y = xlf_of(ny);
                 s.xs = to_std(s.x); s.ys = to_std(s.y);
                 if (a.calc_dc) {
-                    xlf p0 = M[0] * s.P[0] + M[1] * s.P[2];
-                    xlf p1 = M[0] * s.P[1] + M[1] * s.P[3];
-                    xlf p2 = M[2] * s.P[0] + M[3] * s.P[2];
-                    xlf p3 = M[2] * s.P[1] + M[3] * s.P[3];
-                    s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
+                    const xzf P0 = as_zf(s.P[0]), P1 = as_zf(s.P[1]), P2 = as_zf(s.P[2]), P3 = as_zf(s.P[3]);
+                    s.P[0] = xlf_of(mul_w(M[0], P0) + mul_w(M[1], P2));
+                    s.P[1] = xlf_of(mul_w(M[0], P1) + mul_w(M[1], P3));
+                    s.P[2] = xlf_of(mul_w(M[2], P0) + mul_w(M[3], P2));
+                    s.P[3] = xlf_of(mul_w(M[2], P1) + mul_w(M[3], P3));
                 }
                 k.bla++; k.bla_it += st;
             }
@@ -431,13 +433,19 @@ struct BsXr {
         s.n += 1; k.plain++;
         xlf X, Y;
         ZR(t, s.w, X, Y);
+        // one iteration on lazily normalised values, renormalised once
+        const xzf x = as_zf(s.x), y = as_zf(s.y), Xz = as_zf(X), Yz = as_zf(Y);
         if (a.calc_dc) {
-            const xlf z0 = xlf_zero();
-            xlf Rd[4] = {z0, z0, z0, z0};
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = PATH(t, q, s.w);
-            bs_hess<FL>(s.x, s.y, s.P, X, Y, Rd);
+            const xzf z0 = as_zf(xlf_zero());
+            xzf Rd[4] = {z0, z0, z0, z0};
+            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = as_zf(PATH(t, q, s.w));
+            xzf P[4] = {as_zf(s.P[0]), as_zf(s.P[1]), as_zf(s.P[2]), as_zf(s.P[3])};
+            bs_hess<FL>(x, y, P, Xz, Yz, Rd);
+            for (int q = 0; q < 4; q++) s.P[q] = xlf_of(P[q]);
         }
-        bs_zn<FL>(s.x, s.y, X, Y, s.a, s.b);
+        xzf nx = x, ny = y;
+        bs_zn<FL>(nx, ny, Xz, Yz, as_zf(s.a), as_zf(s.b));
+        s.x = xlf_of(nx); s.y = xlf_of(ny);
         s.xs = to_std(s.x); s.ys = to_std(s.y);
         if (s.n >= a.max_iter) return 0;
         s.w += 1;

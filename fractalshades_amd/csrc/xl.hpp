@@ -185,6 +185,60 @@ FSG_DEV xz operator*(xz a, cplx b) {
 FSG_DEV xz muls(double s, xz a) { xz r; r.re = s * a.re; r.im = s * a.im; r.e = a.e; return r; }   // s != 0
 FSG_DEV xlf abs2(xz a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
 
+// Lazily normalised reals for the burning-ship step (the same argument as
+// xz: mantissas of the operands start in [1, 2), one iteration's products
+// of at most three factors and sums of a few terms stay far inside the
+// binary64 range, so each operation rounds as the reference's).  A sum that
+// cancels to zero, or a product by a zero scalar (the folds' 0 / +-1 / +-2
+// derivative factors), takes XL_ZERO_E; a product of two values is zero only
+// when a factor already carries it.
+struct xzf { double m; int32_t e; };
+
+FSG_DEV xzf as_zf(xlf a) { xzf r; r.m = a.m; r.e = a.e; return r; }
+FSG_DEV xlf xlf_of(xzf a) { return xlf_n(a.m, a.e); }
+FSG_DEV xzf zf_mk(double m, int32_t e) { xzf r; r.m = m; r.e = (m == 0.) ? XL_ZERO_E : e; return r; }
+FSG_DEV xzf operator+(xzf a, xzf b) {
+    const int32_t E = max(a.e, b.e);
+    return zf_mk(ldexp(a.m, shift_of(a.e, E)) + ldexp(b.m, shift_of(b.e, E)), E);
+}
+FSG_DEV xzf operator-(xzf a, xzf b) {
+    const int32_t E = max(a.e, b.e);
+    return zf_mk(ldexp(a.m, shift_of(a.e, E)) - ldexp(b.m, shift_of(b.e, E)), E);
+}
+FSG_DEV xzf operator-(xzf a) { a.m = -a.m; return a; }
+FSG_DEV xzf operator*(xzf a, xzf b) { xzf r; r.m = a.m * b.m; r.e = a.e + b.e; return r; }
+FSG_DEV xzf operator*(double s, xzf a) { xzf r; r.m = s * a.m; r.e = (s == 0.) ? XL_ZERO_E : a.e; return r; }
+FSG_DEV xzf mul_s(double s, xzf a) { return s * a; }
+// s * a for a binary64 s of any magnitude (BLA coefficients): s is split
+// into a mantissa in [1, 2) and an exponent, so that sums of such products
+// stay in range; the product rounds as s * a.m does
+FSG_DEV xzf mul_w(double s, xzf a) {
+    const int32_t k = dexp(s) - 1;
+    xzf r;
+    r.m = ldexp(s, -k) * a.m;
+    r.e = (s == 0.) ? XL_ZERO_E : a.e + k;
+    return r;
+}
+FSG_DEV xzf absv(xzf x) { x.m = fabs(x.m); return x; }
+// comparisons with zero: the sign of the mantissa
+FSG_DEV xzf diffabs(xzf X, xzf x) {
+    const xzf Xx = X + x;
+    if (X.m >= 0.) { if (Xx.m >= 0.) return x; return -(2. * X + x); }
+    if (Xx.m <= 0.) return -x;
+    return 2. * X + x;
+}
+FSG_DEV double ddX(xzf X, xzf x) {
+    const xzf Xx = X + x;
+    if (X.m >= 0.) return Xx.m >= 0. ? 0. : -2.;
+    return Xx.m <= 0. ? 0. : 2.;
+}
+FSG_DEV double ddx(xzf X, xzf x) {
+    const xzf Xx = X + x;
+    if (X.m >= 0.) return Xx.m >= 0. ? 1. : -1.;
+    return Xx.m <= 0. ? -1. : 1.;
+}
+FSG_DEV double sgn(xzf x) { return x.m < 0. ? -1. : 1.; }
+
 // burning-ship helpers on Xrange reals (burning_ship.py:19-60)
 FSG_DEV xlf diffabs(xlf X, xlf x) {
     if (ge(X, 0.)) { if (ge(X + x, 0.)) return x; return -(2. * X + x); }
