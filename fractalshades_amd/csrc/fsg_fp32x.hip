@@ -339,6 +339,50 @@ __device__ __forceinline__ double ddx(xsr X, xsr x) {
     return (X + x).m <= 0.f ? -1. : 1.;
 }
 
+// Lazily normalised form of xsr for one iteration (as xzf for binary64):
+// the operands start normalised, the step's products of at most three
+// factors and sums of a few terms stay far inside the binary32 range, and
+// the state is renormalised once per iteration.  A cancelled sum or a
+// product by a zero scalar takes XS_ZERO_E.
+struct xzs { float m; int32_t e; };
+
+__device__ __forceinline__ xzs as_zs(xsr a) { xzs r; r.m = a.m; r.e = a.e; return r; }
+__device__ __forceinline__ xsr xsr_of(xzs a) { return xsr_n(a.m, a.e); }
+__device__ __forceinline__ xzs zs_mk(float m, int32_t e) { xzs r; r.m = m; r.e = (m == 0.f) ? XS_ZERO_E : e; return r; }
+__device__ __forceinline__ xzs operator+(xzs a, xzs b) {
+    const int32_t E = max(a.e, b.e);
+    return zs_mk(ldexpf(a.m, max(a.e - E, -200)) + ldexpf(b.m, max(b.e - E, -200)), E);
+}
+__device__ __forceinline__ xzs operator-(xzs a) { a.m = -a.m; return a; }
+__device__ __forceinline__ xzs operator-(xzs a, xzs b) { return a + (-b); }
+__device__ __forceinline__ xzs operator*(xzs a, xzs b) { xzs r; r.m = a.m * b.m; r.e = a.e + b.e; return r; }
+__device__ __forceinline__ xzs operator*(double s, xzs a) {
+    xzs r; r.m = (float)s * a.m; r.e = (s == 0.) ? XS_ZERO_E : a.e; return r;
+}
+__device__ __forceinline__ xzs mul_s(double s, xzs a) { return s * a; }
+__device__ __forceinline__ xzs mul_w(double s, xzs a) {
+    const int32_t k = __builtin_amdgcn_frexp_exp(s) - 1;
+    xzs r; r.m = (float)ldexp(s, -k) * a.m; r.e = (s == 0.) ? XS_ZERO_E : a.e + k; return r;
+}
+__device__ __forceinline__ xzs absv(xzs x) { x.m = fabsf(x.m); return x; }
+__device__ __forceinline__ double sgn(xzs x) { return x.m < 0.f ? -1. : 1.; }
+__device__ __forceinline__ xzs diffabs(xzs X, xzs x) {
+    const xzs Xx = X + x;
+    if (X.m >= 0.f) { if (Xx.m >= 0.f) return x; return -(2. * X + x); }
+    if (Xx.m <= 0.f) return -x;
+    return 2. * X + x;
+}
+__device__ __forceinline__ double ddX(xzs X, xzs x) {
+    const xzs Xx = X + x;
+    if (X.m >= 0.f) return Xx.m >= 0.f ? 0. : -2.;
+    return Xx.m <= 0.f ? 0. : 2.;
+}
+__device__ __forceinline__ double ddx(xzs X, xzs x) {
+    const xzs Xx = X + x;
+    if (X.m >= 0.f) return Xx.m >= 0.f ? 1. : -1.;
+    return Xx.m <= 0.f ? -1. : 1.;
+}
+
 template <int FL>
 struct BsF32 {
     struct State { xsr x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; };
@@ -369,16 +413,16 @@ struct BsF32 {
             {
                 const double *M = t.M + 8 * idx;
                 s.n += st; s.w += st;
-                const xsr nx = mul_w(M[0], s.x) + mul_w(M[1], s.y) + mul_w(M[4], s.a) + mul_w(M[5], s.b);
-                const xsr ny = mul_w(M[2], s.x) + mul_w(M[3], s.y) + mul_w(M[6], s.a) + mul_w(M[7], s.b);
-                s.x = nx; s.y = ny;
+                const xzs x = as_zs(s.x), y = as_zs(s.y), ca = as_zs(s.a), cb = as_zs(s.b);
+                s.x = xsr_of(mul_w(M[0], x) + mul_w(M[1], y) + mul_w(M[4], ca) + mul_w(M[5], cb));
+                s.y = xsr_of(mul_w(M[2], x) + mul_w(M[3], y) + mul_w(M[6], ca) + mul_w(M[7], cb));
                 s.xs = xsr_to_d(s.x); s.ys = xsr_to_d(s.y);
                 if (a.calc_dc) {
-                    const xsr p0 = mul_w(M[0], s.P[0]) + mul_w(M[1], s.P[2]);
-                    const xsr p1 = mul_w(M[0], s.P[1]) + mul_w(M[1], s.P[3]);
-                    const xsr p2 = mul_w(M[2], s.P[0]) + mul_w(M[3], s.P[2]);
-                    const xsr p3 = mul_w(M[2], s.P[1]) + mul_w(M[3], s.P[3]);
-                    s.P[0] = p0; s.P[1] = p1; s.P[2] = p2; s.P[3] = p3;
+                    const xzs P0 = as_zs(s.P[0]), P1 = as_zs(s.P[1]), P2 = as_zs(s.P[2]), P3 = as_zs(s.P[3]);
+                    s.P[0] = xsr_of(mul_w(M[0], P0) + mul_w(M[1], P2));
+                    s.P[1] = xsr_of(mul_w(M[0], P1) + mul_w(M[1], P3));
+                    s.P[2] = xsr_of(mul_w(M[2], P0) + mul_w(M[3], P2));
+                    s.P[3] = xsr_of(mul_w(M[2], P1) + mul_w(M[3], P3));
                 }
                 k.bla++; k.bla_it += st;
             }
@@ -386,13 +430,19 @@ struct BsF32 {
         s.n += 1; k.plain++;
         xsr X, Y;
         ZR(t, s.w, X, Y);
+        // one iteration on lazily normalised values, renormalised once
+        const xzs x = as_zs(s.x), y = as_zs(s.y), Xz = as_zs(X), Yz = as_zs(Y);
         if (a.calc_dc) {
-            const xsr z0 = xsr{0.f, XS_ZERO_E};
-            xsr Rd[4] = {z0, z0, z0, z0};
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = PATH(t, q, s.w);
-            bs_hess<FL>(s.x, s.y, s.P, X, Y, Rd);
+            const xzs z0 = xzs{0.f, XS_ZERO_E};
+            xzs Rd[4] = {z0, z0, z0, z0};
+            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = as_zs(PATH(t, q, s.w));
+            xzs P[4] = {as_zs(s.P[0]), as_zs(s.P[1]), as_zs(s.P[2]), as_zs(s.P[3])};
+            bs_hess<FL>(x, y, P, Xz, Yz, Rd);
+            for (int q = 0; q < 4; q++) s.P[q] = xsr_of(P[q]);
         }
-        bs_zn<FL>(s.x, s.y, X, Y, s.a, s.b);
+        xzs nx = x, ny = y;
+        bs_zn<FL>(nx, ny, Xz, Yz, as_zs(s.a), as_zs(s.b));
+        s.x = xsr_of(nx); s.y = xsr_of(ny);
         s.xs = xsr_to_d(s.x); s.ys = xsr_to_d(s.y);
         if (s.n >= a.max_iter) return 0;
         s.w += 1;
