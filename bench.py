@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 import numpy as np
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector / matrix peak
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 
 # HBM bytes per launch of the binary64 tile kernel on workload C2, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
@@ -294,8 +295,11 @@ class Leg:
                 self.d_si.cpu().numpy().reshape(1, n))
 
     def summary(self):
+        # the binary32 form's flops are priced against the FP32 vector peak
+        peak = FP32_VECTOR_PEAK_TFLOPS if self.fields["precision"] == 1 else FP64_VECTOR_PEAK_TFLOPS
         return {"value": self.value, "ms_per_step": self.ms_per_step, "kernel_ms": self.kernel_ms,
                 "pixel_iters_per_step": self.pix_iters, "fp64_equiv_tflops": self.tflops,
+                "vector_peak_tflops": peak, "vector_peak_frac": self.tflops / peak,
                 "counts": self.stats,
                 "stop_reason_hist": dict(zip(["-1", "max_iter", "divergence", "stationnary"], self.reasons))}
 
