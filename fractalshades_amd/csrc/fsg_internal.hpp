@@ -79,7 +79,9 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
 // (fsg_lds.hip, 1024-lane blocks)
 typedef void (*tile_fn)(const KArgs);
 tile_fn select_tile_orbit(const LaunchCfg &cfg);
-tile_fn select_tile_lds(const LaunchCfg &cfg);
+tile_fn select_tile_lds(const LaunchCfg &cfg, bool full);
+// true when the plan stages every table the launch reads
+bool lds_plan_full(const Tables &t, const LdsPlan &p);
 // radii_only: M already holds the coefficients (shared with another
 // context of the same orbit); write the radii r only
 hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, int32_t L,

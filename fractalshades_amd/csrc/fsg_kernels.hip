@@ -276,6 +276,14 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     return p;
 }
 
+bool lds_plan_full(const Tables &t, const LdsPlan &p) {
+    bool ok = p.zn >= 0 && (!t.xr_pos || p.xr_pos >= 0) && (!t.dz || p.dz >= 0) &&
+              (!t.dz_e || p.dz_e >= 0) && (!t.dc || p.dc >= 0) && (!t.dc_e || p.dc_e >= 0) &&
+              (!t.bla_on || (p.r >= 0 && p.M >= 0));
+    for (int q = 0; q < 4; q++) ok = ok && (!t.bs[q] || p.bs[q] >= 0) && (!t.bs_e[q] || p.bs_e[q] >= 0);
+    return ok;
+}
+
 hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
     // FSG_TABLES=global keeps the tables in global memory (for comparisons)
     const char *tables = std::getenv("FSG_TABLES");
@@ -283,7 +291,7 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
     if (!cfg.orbit && !no_lds) {
         KArgs al = a;
         al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());
-        tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg) : nullptr;
+        tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg, lds_plan_full(a.t, al.lds)) : nullptr;
         int per_cu = 0, dev = 0, ncu = 0;
         if (f && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&

@@ -5,13 +5,15 @@
 
 namespace fsg {
 
-template <class Model> static tile_fn fn_lds() { return k_tile_lds<Model>; }
+template <class Model> static tile_fn fn_lds(bool full) {
+    return full ? k_tile_lds<Model, true> : k_tile_lds<Model, false>;
+}
 
-tile_fn select_tile_lds(const LaunchCfg &c) {
-    if (c.kind == KIND_M2) return c.xr ? fn_lds<HoloXr<KIND_M2, 2>>() : fn_lds<HoloStd<KIND_M2, 2>>();
+tile_fn select_tile_lds(const LaunchCfg &c, bool full) {
+    if (c.kind == KIND_M2) return c.xr ? fn_lds<HoloXr<KIND_M2, 2>>(full) : fn_lds<HoloStd<KIND_M2, 2>>(full);
     if (c.kind == KIND_MN) {
         switch (c.degree) {
-#define FSG_MN(D) case D: return c.xr ? fn_lds<HoloXr<KIND_MN, D>>() : fn_lds<HoloStd<KIND_MN, D>>();
+#define FSG_MN(D) case D: return c.xr ? fn_lds<HoloXr<KIND_MN, D>>(full) : fn_lds<HoloStd<KIND_MN, D>>(full);
             FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
 #undef FSG_MN
             default: return nullptr;
@@ -19,7 +21,7 @@ tile_fn select_tile_lds(const LaunchCfg &c) {
     }
     if (c.kind == KIND_BS) {
         switch (c.flavor) {
-#define FSG_BS(F) case F: return c.xr ? fn_lds<BsXr<F>>() : fn_lds<BsStd<F>>();
+#define FSG_BS(F) case F: return c.xr ? fn_lds<BsXr<F>>(full) : fn_lds<BsStd<F>>(full);
             FSG_BS(1) FSG_BS(2) FSG_BS(3) FSG_BS(4) FSG_BS(5)
 #undef FSG_BS
             default: return nullptr;

@@ -163,44 +163,55 @@ __global__ void __launch_bounds__(256) k_tile(const KArgs a) {
 // radius and coefficients at the lane's own iteration index), and an LDS
 // read answers in a fraction of an L2 hit's time.  One 1024-lane block per
 // CU (or more when they fit) copies the tables once, then runs the loop.
-template <class T>
+// FULL: the plan stages every table the launch has, so each table pointer
+// is either null or an LDS address, and the compiler reads the tables with
+// LDS instructions instead of generic (flat) ones
+template <bool FULL, class T>
 __device__ __forceinline__ const T *stage(unsigned char *lds, int32_t off, const T *src, int64_t n) {
-    if (off < 0 || src == nullptr) return src;
+    if constexpr (FULL) {
+        if (src == nullptr || off < 0) return nullptr;   // off < 0: a table the launch never reads
+    } else {
+        if (off < 0 || src == nullptr) return src;
+    }
     T *dst = reinterpret_cast<T *>(lds + off);
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
     return dst;
 }
 
-template <class Model>
+template <class Model, bool FULL>
 __global__ void __launch_bounds__(1024) k_tile_lds(const KArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsPlan &p = a.lds;
     KArgs al = a;
     Tables &t = al.t;
     const int64_t K = p.K, KB = p.KB;
-    t.Zn = stage(lds, p.zn, a.t.Zn, K);
-    t.dc = stage(lds, p.dc, a.t.dc, K);
-    t.dc_e = stage(lds, p.dc_e, a.t.dc_e, K);
-    if (p.dz >= 0 && a.t.dz) {
+    t.Zn = stage<FULL>(lds, p.zn, a.t.Zn, K);
+    t.dc = stage<FULL>(lds, p.dc, a.t.dc, K);
+    t.dc_e = stage<FULL>(lds, p.dc_e, a.t.dc_e, K);
+    if ((FULL || p.dz >= 0) && a.t.dz) {
         cplx *dz = reinterpret_cast<cplx *>(lds + p.dz);
         for (int64_t i = threadIdx.x; i < K; i += blockDim.x) dz[i] = a.t.dz[i];
         if (threadIdx.x == 0) dz[K] = a.t.dz[a.t.L];
         t.dz = dz;
+    } else if constexpr (FULL) {
+        t.dz = nullptr;
     }
-    if (p.dz_e >= 0 && a.t.dz_e) {
+    if ((FULL || p.dz_e >= 0) && a.t.dz_e) {
         int32_t *dze = reinterpret_cast<int32_t *>(lds + p.dz_e);
         for (int64_t i = threadIdx.x; i < K; i += blockDim.x) dze[i] = a.t.dz_e[i];
         if (threadIdx.x == 0) dze[K] = a.t.dz_e[a.t.L];
         t.dz_e = dze;
+    } else if constexpr (FULL) {
+        t.dz_e = nullptr;
     }
     for (int q = 0; q < 4; q++) {
-        t.bs[q] = stage(lds, p.bs[q], a.t.bs[q], K);
-        t.bs_e[q] = stage(lds, p.bs_e[q], a.t.bs_e[q], K);
+        t.bs[q] = stage<FULL>(lds, p.bs[q], a.t.bs[q], K);
+        t.bs_e[q] = stage<FULL>(lds, p.bs_e[q], a.t.bs_e[q], K);
     }
-    t.xr_pos = stage(lds, p.xr_pos, a.t.xr_pos, K);
-    t.r = stage(lds, p.r, a.t.r, KB);
-    t.M = stage(lds, p.M, a.t.M, KB * p.bla_dim);
-    if (p.dz >= 0 && a.t.dz) t.L = (int32_t)K;   // the wrapped dZndz value is staged at [K]
+    t.xr_pos = stage<FULL>(lds, p.xr_pos, a.t.xr_pos, K);
+    t.r = stage<FULL>(lds, p.r, a.t.r, KB);
+    t.M = stage<FULL>(lds, p.M, a.t.M, KB * p.bla_dim);
+    if ((FULL || p.dz >= 0) && a.t.dz) t.L = (int32_t)K;   // the wrapped dZndz value is staged at [K]
     __syncthreads();
     tile_loop<Model>(al);
 }
