@@ -20,6 +20,8 @@ descriptor of the same two layers computed on the GPU (fsg_post_desc).
 """
 import numpy as np
 
+from .storage import Fractal_array   # postproc.py:829-960  # noqa: F401
+
 
 def continuous_iter(zn, stop_iter, d=2, a_d=1., M=1000., floor_iter=0):
     """Continuous iteration number nu for every pixel."""

@@ -19,10 +19,17 @@ The reference keeps, per calculation, under ``<directory>/data/``
 
 ``calc_to_disk`` computes the tiles whose ``done`` flag is 0 -- all of them
 in one launch, with each tile's own pixel grid as the reference computes it
--- and writes them in place.  Calculations with a ``subset`` are not
-supported (the reference stores the subset mask per tile as well).  The
-reference orbit file ``ref_pt.dat`` is not written: its Xrange register holds
-objects of the reference's own Xrange class.
+-- and writes them in place.
+
+A calculation with a ``subset`` (usually a ``Fractal_array`` over another
+calculation's stored results, postproc.py:829-960) stores only the pixels
+of the mask: the points of a tile and the arrays' second dimension count
+the selected pixels, and ``<calc>_subset.arr`` holds the mask itself, a
+bool memmap of nx * ny entries, tile after tile (core.py:1745-1762,
+2404-2424, 2490-2507).
+
+The reference orbit file ``ref_pt.dat`` is not written: its Xrange register
+holds objects of the reference's own Xrange class.
 """
 import os
 import pickle
@@ -83,21 +90,106 @@ def fingerprint_matching(f, calc_name):
 
 
 def _same(a, b):
+    if isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+        return isinstance(a, np.ndarray) and isinstance(b, np.ndarray) and np.array_equal(a, b)
     try:
         return bool(a == b)
     except Exception:
         return False
 
 
+class Fractal_array:
+    """One stored field of a calculation, tile by tile (postproc.py:829-960):
+    ``arr[chunk_slice]`` is the field over the tile's stored points (all of
+    them with ``chunk_slice`` None), with ``func`` applied (a callable or an
+    expression in ``x``, e.g. ``"x != 1"``) and ``~`` inverting a boolean
+    result.  Passed as the ``subset`` of another calculation, it selects
+    the pixels that calculation computes."""
+
+    def __init__(self, fractal, calc_name, key, func=None, inv=False):
+        self.fractal = fractal
+        self.calc_name = calc_name
+        self.key = key
+        self._func = func
+        self.func = self.parse_func(func)
+        self.inv = inv
+
+    @staticmethod
+    def parse_func(func):
+        if isinstance(func, str):
+            code = compile(func, "<Fractal_array func>", "eval")
+            return lambda x: eval(code, {"np": np, "__builtins__": {}}, {"x": x})
+        return func
+
+    def __reduce__(self):
+        # the fractal is not part of the pickled state (it holds device
+        # contexts); an unpickled array compares equal and is rebound with
+        # bind_fractal
+        return (self.__class__, (None, self.calc_name, self.key, self._func, self.inv))
+
+    def bind_fractal(self, fractal):
+        self.fractal = fractal
+
+    def __eq__(self, other):
+        return (isinstance(other, Fractal_array) and self._func == other._func
+                and self.calc_name == other.calc_name and self.key == other.key
+                and self.inv == other.inv)
+
+    def __invert__(self):
+        return Fractal_array(self.fractal, self.calc_name, self.key, self._func, not self.inv)
+
+    def _field(self):
+        codes = self.fractal._calc_data[self.calc_name]["state"].codes
+        complex_codes, int_codes, _ = codes
+        if self.key in complex_codes:
+            return "Z", complex_codes.index(self.key)
+        if self.key in int_codes:
+            return "U", int_codes.index(self.key)
+        if self.key in ("stop_reason", "stop_iter"):
+            return self.key, 0
+        raise ValueError(f"unknown key {self.key!r} for {self.calc_name}")
+
+    def __getitem__(self, chunk_slice):
+        f = self.fractal
+        if f is None:
+            raise ValueError("Fractal_array without a fractal: use bind_fractal")
+        arr_key, row = self._field()
+        mm = np.load(data_path(f, self.calc_name)[arr_key], mmap_mode="r")
+        if chunk_slice is None:
+            arr = np.array(mm[row, :])
+        else:
+            rep = np.load(report_path(f, self.calc_name), mmap_mode="r")
+            r = f.chunk_rank(chunk_slice)
+            beg = int(rep[r, REPORT_ITEMS.index("chunk1d_begin")])
+            end = int(rep[r, REPORT_ITEMS.index("chunk1d_end")])
+            arr = np.array(mm[row, beg:end])
+        if self.func is not None:
+            arr = self.func(arr)
+        if self.inv:
+            arr = ~arr
+        return arr
+
+
 def _state(f, calc_name):
-    state = f._calc_data[calc_name]["state"]
-    if getattr(state, "subset", None) is not None:
-        raise NotImplementedError("on-disk results of a calculation with a subset")
-    return state
+    return f._calc_data[calc_name]["state"]
 
 
-def _chunk_pts(f):
+def _mask(subset, cs):
+    return np.ravel(np.asarray(subset[cs], dtype=bool))
+
+
+def _full_pts(f):
+    """Pixels of each tile, in chunk order (the uncompressed layout)."""
     return np.array([(ixx - ix) * (iyy - iy) for ix, ixx, iy, iyy in f.chunk_slices()], dtype=np.int64)
+
+
+def _chunk_pts(f, state=None):
+    """Stored points of each tile: the selected pixels under a subset
+    (core.py:1745-1762)."""
+    subset = getattr(state, "subset", None)
+    if subset is None:
+        return _full_pts(f)
+    return np.array([int(np.count_nonzero(_mask(subset, cs))) for cs in f.chunk_slices()], dtype=np.int64)
 
 
 def init_files(f, calc_name):
@@ -107,7 +199,7 @@ def init_files(f, calc_name):
     os.makedirs(os.path.join(f.directory, "data"), exist_ok=True)
     with open(fingerprint_path(f, calc_name), "wb") as fh:
         pickle.dump(fingerprint(f, calc_name), fh, pickle.HIGHEST_PROTOCOL)
-    pts = _chunk_pts(f)
+    pts = _chunk_pts(f, state)
     cum = np.zeros(len(pts) + 1, dtype=np.int64)
     np.cumsum(pts, out=cum[1:])
     rep = open_memmap(report_path(f, calc_name), mode="w+", dtype=np.int32, shape=(len(pts), len(REPORT_ITEMS)))
@@ -124,6 +216,16 @@ def init_files(f, calc_name):
     for key in SAVE_ARRS:
         shape, dt = dims[key]
         mm = open_memmap(paths[key], mode="w+", dtype=np.dtype(dt), shape=shape)
+        del mm
+    subset = getattr(state, "subset", None)
+    if subset is not None:
+        # the mask, fixed size whatever it selects (core.py:2404-2424)
+        full = _full_pts(f)
+        ucum = np.concatenate([[0], np.cumsum(full)])
+        mm = open_memmap(paths["subset"], mode="w+", dtype=bool, shape=(int(ucum[-1]),))
+        for r, cs in enumerate(f.chunk_slices()):
+            mm[ucum[r]:ucum[r + 1]] = _mask(subset, cs)
+        mm.flush()
         del mm
 
 
@@ -152,9 +254,15 @@ def calc_to_disk(f, calc_name):
             if rep[r, REPORT_ITEMS.index("done")] == 0]
     if not todo:
         return 0
-    # every tile's own pixel grid, in the stored order, in one launch
-    c_pix = np.concatenate([np.ravel(f.chunk_pixel_pos(cs)) for _, cs in todo])
+    # every tile's own pixel grid (its selected pixels under a subset), in
+    # the stored order, in one launch
     state = f._calc_data[calc_name]["state"]
+    subset = getattr(state, "subset", None)
+    parts = []
+    for _, cs in todo:
+        c = np.ravel(f.chunk_pixel_pos(cs))
+        parts.append(c if subset is None else c[_mask(subset, cs)])
+    c_pix = np.concatenate(parts)
     args = f._result_arrays(state, c_pix)
     ret = f.numba_cycle_call(args, f._calc_data[calc_name]["cycle_indep_args"])
     if ret != 0:
@@ -182,8 +290,8 @@ def calc_to_disk(f, calc_name):
 
 def reload_data(f, chunk_slice, calc_name):
     """(subset, c_pix, Z, U, stop_reason, stop_iter) of one stored tile
-    (core.py:2474-2512)."""
-    _state(f, calc_name)
+    (core.py:2474-2512); subset is the tile's mask, or None."""
+    state = _state(f, calc_name)
     rep = np.load(report_path(f, calc_name), mmap_mode="r")
     r = f.chunk_rank(chunk_slice)
     beg = int(rep[r, REPORT_ITEMS.index("chunk1d_begin")])
@@ -191,4 +299,10 @@ def reload_data(f, chunk_slice, calc_name):
     paths = data_path(f, calc_name)
     arr = {key: np.load(paths[key], mmap_mode="r")[:, beg:end] for key in SAVE_ARRS}
     c_pix = np.ravel(f.chunk_pixel_pos(chunk_slice))
-    return None, c_pix, arr["Z"], arr["U"], arr["stop_reason"], arr["stop_iter"]
+    mask = None
+    if getattr(state, "subset", None) is not None:
+        full = _full_pts(f)
+        ubeg = int(full[:r].sum())
+        mask = np.array(np.load(paths["subset"], mmap_mode="r")[ubeg:ubeg + int(full[r])])
+        c_pix = c_pix[mask]
+    return mask, c_pix, arr["Z"], arr["U"], arr["stop_reason"], arr["stop_iter"]

@@ -46,11 +46,57 @@ def test_layout_and_fingerprint(tmp_path):
     assert not storage.res_available(g, "t")
 
 
-def test_subset_not_supported(tmp_path):
-    f = _frame(tmp_path, nx=16)
-    f._calc_data["t"]["state"].subset = np.ones(f.nx * f.ny, dtype=bool)
-    with pytest.raises(NotImplementedError):
-        storage.init_files(f, "t")
+def _add_subset_calc(f, name="t2", func="x == 1", key="stop_reason"):
+    """A second calculation on the frame, over the pixels selected by a
+    Fractal_array of the stored calculation "t"."""
+    sub = storage.Fractal_array(f, "t", key, func=func)
+    kw = dict(f._calc_data["t"]["calc_kwargs"])
+    kw.update(calc_name=name, subset=sub)
+    getattr(f, f._calc_data["t"]["calc_callable"])(**kw)
+    return sub
+
+
+def test_subset_layout(tmp_path):
+    """A calculation over a Fractal_array subset stores only the selected
+    points: compressed report and arrays, the mask in <calc>_subset.arr,
+    tile after tile (core.py:1745-1762, 2404-2424)."""
+    f = _frame(tmp_path)
+    storage.init_files(f, "t")
+    # a stored stop_reason pattern for "t", as if computed
+    sr = np.lib.format.open_memmap(storage.data_path(f, "t")["stop_reason"], mode="r+")
+    rng = np.random.default_rng(0)
+    pattern = rng.integers(0, 3, sr.shape[1]).astype(np.int8)
+    sr[0, :] = pattern
+    del sr
+    sub = _add_subset_calc(f)
+    rep_t = np.load(storage.report_path(f, "t"))
+    masks = []
+    for r, cs in enumerate(f.chunk_slices()):
+        m = sub[cs]
+        want = pattern[rep_t[r, 0]:rep_t[r, 1]] == 1
+        np.testing.assert_array_equal(m, want)
+        masks.append(want)
+    np.testing.assert_array_equal(sub[None], pattern == 1)
+    np.testing.assert_array_equal((~sub)[next(iter(f.chunk_slices()))], ~masks[0])
+    storage.init_files(f, "t2")
+    rep = np.load(storage.report_path(f, "t2"))
+    counts = [int(m.sum()) for m in masks]
+    assert list(rep[:, 2]) == counts and rep[-1, 1] == sum(counts)
+    z = np.load(storage.data_path(f, "t2")["Z"], mmap_mode="r")
+    assert z.shape[1] == sum(counts)
+    mask_file = np.load(storage.data_path(f, "t2")["subset"])
+    assert mask_file.dtype == bool and mask_file.shape == (f.nx * f.ny,)
+    np.testing.assert_array_equal(mask_file, np.concatenate(masks))
+    # reload: the tile's mask and its selected pixel offsets
+    cs = list(f.chunk_slices())[2]
+    m, c_pix, Z, U, sr2, si = storage.reload_data(f, cs, "t2")
+    np.testing.assert_array_equal(m, masks[2])
+    np.testing.assert_array_equal(c_pix, np.ravel(f.chunk_pixel_pos(cs))[masks[2]])
+    assert Z.shape[1] == counts[2]
+    # the fingerprint keeps the subset and matches after a reload from disk
+    assert storage.fingerprint_matching(f, "t2")
+    g_sub = storage.Fractal_array(f, "t", "stop_reason", func="x == 2")
+    assert g_sub != sub and storage.Fractal_array(f, "t", "stop_reason", func="x == 1") == sub
 
 
 def _tiles_equal_one_by_one(f):
@@ -90,3 +136,33 @@ def test_resume_computes_only_missing_tiles(tmp_path):
     del z
     assert storage.calc_to_disk(f, "t") == 2
     _tiles_equal_one_by_one(f)
+
+
+@pytest.mark.gpu
+def test_subset_calc_to_disk(tmp_path):
+    """A calculation over the pixels of a stored one with an even escape
+    count: its stored tiles equal the tiles computed one by one with the
+    same subset, and the stored calculation's results at the selected
+    pixels."""
+    f = _frame(tmp_path)
+    print("case storage subset", flush=True)
+    storage.calc_to_disk(f, "t")
+    _add_subset_calc(f, key="stop_iter", func="x % 2 == 0")
+    assert storage.calc_to_disk(f, "t2") == 4
+    n_sel = 0
+    for cs in f.chunk_slices():
+        m, c_pix, Z, U, sr, si = storage.reload_data(f, cs, "t2")
+        ret, (c1, Z1, U1, sr1, si1) = f.compute_tile("t2", cs)
+        assert ret == 0
+        np.testing.assert_array_equal(c_pix, c1)
+        for a, b in ((Z, Z1), (U, U1), (sr, sr1), (si, si1)):
+            np.testing.assert_array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+        _, _, Zt, Ut, srt, sit = storage.reload_data(f, cs, "t")
+        sel = np.ascontiguousarray(np.asarray(Zt)[:, m])
+        np.testing.assert_array_equal(np.ascontiguousarray(Z).view(np.uint8), sel.view(np.uint8))
+        for a, b in ((U, Ut), (sr, srt), (si, sit)):
+            np.testing.assert_array_equal(a, np.asarray(b)[:, m])
+        assert np.all(np.asarray(si) % 2 == 0)
+        n_sel += int(m.sum())
+    assert 0 < n_sel < f.nx * f.ny
+    assert storage.calc_to_disk(f, "t2") == 0
