@@ -176,6 +176,18 @@ class Fractal:
             c_pix = c_pix[chunk_subset]
         return self._result_arrays(state, c_pix), chunk_subset
 
+    def _image_mask(self, subset):
+        """A subset as a mask over the whole frame, row-major (y, x): an
+        array is taken as it is; a tile-indexed subset (a Fractal_array)
+        is assembled from its tiles."""
+        if isinstance(subset, np.ndarray):
+            return np.ravel(subset.astype(bool))
+        mask = np.zeros((self.ny, self.nx), dtype=bool)
+        for cs in self.chunk_slices():
+            ix, ixx, iy, iyy = cs
+            mask[iy:iyy, ix:ixx] = np.asarray(subset[cs], dtype=bool).reshape(iyy - iy, ixx - ix)
+        return np.ravel(mask)
+
     def _result_arrays(self, state, c_pix):
         n_pts = c_pix.shape[0]
         n_Z, n_U, _ = (len(code) for code in state.codes)
@@ -210,7 +222,7 @@ class Fractal:
         subset = getattr(state, "subset", None)
         grid_w = self.nx * (supersampling or 1)
         if subset is not None:
-            c_pix = c_pix[np.ravel(np.asarray(subset, dtype=bool))]
+            c_pix = c_pix[self._image_mask(subset)]
             grid_w = 0
         args = self._result_arrays(state, c_pix)
         ret = self.numba_cycle_call(args, self._calc_data[calc_name]["cycle_indep_args"],

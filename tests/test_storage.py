@@ -166,3 +166,34 @@ def test_subset_calc_to_disk(tmp_path):
         n_sel += int(m.sum())
     assert 0 < n_sel < f.nx * f.ny
     assert storage.calc_to_disk(f, "t2") == 0
+    # the whole frame at once with the same subset (calc_raw assembles the
+    # mask from the tiles): the full frame's results at the selected pixels
+    full = f.calc_raw("t")
+    part = f.calc_raw("t2")
+    mask = f._image_mask(f._calc_data["t2"]["state"].subset)
+    assert part["stop_iter"].shape[1] == n_sel == int(mask.sum())
+    np.testing.assert_array_equal(part["stop_iter"], full["stop_iter"][:, mask])
+    np.testing.assert_array_equal(np.ascontiguousarray(part["Z"]).view(np.uint8),
+                                  np.ascontiguousarray(full["Z"][:, mask]).view(np.uint8))
+
+
+def test_image_mask_from_tiles(tmp_path):
+    """A tile-indexed subset assembled into a whole-frame mask: each tile's
+    pixels (chunk_pixel_pos, row-major (y, x)) sit at [iy:iyy, ix:ixx] of
+    the frame's own grid."""
+    f = _frame(tmp_path)
+    full = np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))).reshape(f.ny, f.nx)
+    pattern = np.random.default_rng(1).random((f.ny, f.nx)) < 0.3
+
+    class TileSubset:
+        def __getitem__(self, cs):
+            ix, ixx, iy, iyy = cs
+            return np.ravel(pattern[iy:iyy, ix:ixx])
+
+    for cs in f.chunk_slices():
+        # (a tile's own linspace may differ from the frame's in the last bit)
+        ix, ixx, iy, iyy = cs
+        np.testing.assert_allclose(np.ravel(f.chunk_pixel_pos(cs)), np.ravel(full[iy:iyy, ix:ixx]),
+                                   rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(f._image_mask(TileSubset()), np.ravel(pattern))
+    np.testing.assert_array_equal(f._image_mask(pattern.ravel()), np.ravel(pattern))
