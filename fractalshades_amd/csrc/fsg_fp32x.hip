@@ -13,6 +13,8 @@
 //
 // The loop structure, BLA lookup and rebase rules are those of the binary64
 // kernels (numba_iterate, src/fractalshades/perturbation.py:1065-1400).
+#include <cstring>
+#include <cstdlib>
 #include "models.hpp"
 #include "tile_loop.hpp"
 
@@ -511,7 +513,90 @@ static tile_fn select_f32(const LaunchCfg &c) {
     return nullptr;
 }
 
+// The holomorphic binary32 loop with every table it reads staged in LDS
+// (the binary64 k_tile_lds<Model, true> for this form's tables): binary64
+// orbit and BLA radii for the tests, xs orbit, derivative paths (the
+// wrapped dZndz value at [K]) and BLA coefficients.
+template <class Model>
+__global__ void __launch_bounds__(1024) k_tile_lds_x(const KArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsPlanX &p = a.ldsx;
+    KArgs al = a;
+    Tables &t = al.t;
+    TablesX &x = al.tx;
+    const int64_t K = p.K, KB = p.KB;
+    t.Zn = stage<true>(lds, p.zn, a.t.Zn, K);
+    t.r = stage<true>(lds, p.r, a.t.r, KB);
+    x.Zx = stage<true>(lds, p.zx, static_cast<const xs *>(a.tx.Zx), K);
+    x.DCx = stage<true>(lds, p.dcx, static_cast<const xs *>(a.tx.DCx), K);
+    x.Ax = stage<true>(lds, p.ax, static_cast<const xs *>(a.tx.Ax), KB);
+    x.Bx = stage<true>(lds, p.bx, static_cast<const xs *>(a.tx.Bx), KB);
+    if (a.tx.DZx) {
+        const xs *src = static_cast<const xs *>(a.tx.DZx);
+        xs *dz = reinterpret_cast<xs *>(lds + p.dzx);
+        for (int64_t i = threadIdx.x; i < K; i += blockDim.x) dz[i] = src[i];
+        if (threadIdx.x == 0) dz[K] = src[a.t.L];
+        x.DZx = dz;
+        t.L = (int32_t)K;
+    } else {
+        x.DZx = nullptr;
+    }
+    __syncthreads();
+    tile_loop<Model>(al);
+}
+
+static LdsPlanX plan_lds_x(const Tables &t, const TablesX &x, int max_bytes) {
+    LdsPlanX p;
+    std::memset(&p, 0xff, sizeof(p));
+    int64_t K = std::min<int64_t>(t.L, std::min<int64_t>((int64_t)t.div_iter + 1, t.order));
+    K = std::max<int64_t>(K, std::min<int64_t>(t.L, 2));
+    const int64_t bla_len = 2 * (int64_t)(t.L >> 3);
+    const int64_t KB = (t.bla_on && x.Ax) ? std::min<int64_t>(bla_len, K / 4 + 2) : 0;
+    int64_t off = 0;
+    const auto take = [&](int64_t bytes) { int64_t o = off; off += (bytes + 15) & ~(int64_t)15; return (int32_t)o; };
+    p.zn = take(16 * K);
+    p.zx = take((int64_t)XS_BYTES * K);
+    if (x.DCx) p.dcx = take((int64_t)XS_BYTES * K);
+    if (x.DZx) p.dzx = take((int64_t)XS_BYTES * (K + 1));
+    if (KB > 0) { p.r = take(8 * KB); p.ax = take((int64_t)XS_BYTES * KB); p.bx = take((int64_t)XS_BYTES * KB); }
+    p.K = (int32_t)K; p.KB = (int32_t)KB;
+    p.bytes = (off <= max_bytes && (!t.bla_on || KB > 0)) ? (int32_t)off : 0;
+    return p;
+}
+
+template <class Model> static tile_fn fn_lds_x() { return k_tile_lds_x<Model>; }
+
+static tile_fn select_f32_lds(const LaunchCfg &c) {
+    if (c.kind == KIND_M2) return fn_lds_x<HoloF32<KIND_M2, 2>>();
+    if (c.kind == KIND_MN) {
+        switch (c.degree) {
+#define FSG_MN(D) case D: return fn_lds_x<HoloF32<KIND_MN, D>>();
+            FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
+#undef FSG_MN
+            default: return nullptr;
+        }
+    }
+    return nullptr;
+}
+
 hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream) {
+    // FSG_TABLES=global keeps the tables in global memory (for comparisons)
+    const char *tables = std::getenv("FSG_TABLES");
+    const bool no_lds = tables && !std::strcmp(tables, "global");
+    if (cfg.kind != KIND_BS && !cfg.orbit && !no_lds) {
+        KArgs al = a;
+        al.ldsx = plan_lds_x(a.t, a.tx, max_lds_bytes());
+        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg) : nullptr;
+        int per_cu = 0, dev = 0, ncu = 0;
+        if (fl && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fl), 1024,
+                                                         (size_t)al.ldsx.bytes) == hipSuccess && per_cu > 0) {
+            hipLaunchKernelGGL(fl, dim3(launch_blocks(ncu * per_cu, al, 1024)), dim3(1024),
+                               (size_t)al.ldsx.bytes, stream, al);
+            return hipGetLastError();
+        }
+    }
     tile_fn f = select_f32(cfg);
     if (!f) return hipErrorInvalidValue;
     const int fill = grid_fill_blocks(reinterpret_cast<const void *>(f));

@@ -44,6 +44,11 @@ struct LdsPlan {
     int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, r, M;
 };
 
+// The binary32 form's tables staged in LDS (byte offsets, -1 = absent):
+// orbit (binary64, for the escape tests), BLA radii, and the xs orbit,
+// derivative paths and BLA coefficients; all of them or none.
+struct LdsPlanX { int32_t K, KB, bytes, zn, r, zx, dcx, dzx, ax, bx; };
+
 // One launch of the tile kernel.
 struct KArgs {
     Tables t;
@@ -67,6 +72,7 @@ struct KArgs {
     int32_t nz; int32_t max_blocks;   // grid cap: 0 fill the GPU, > 0 blocks, -k 1/k of the GPU
     int32_t calc_orbit, backshift;    // orbit point at exit - backshift in the last field(s)
     LdsPlan lds;                      // lds.bytes == 0: tables read from global memory
+    LdsPlanX ldsx;                    // the same for the binary32 form
     // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
     // rebase, bla lookups)
     unsigned long long *work;
@@ -92,6 +98,7 @@ int grid_fill_blocks(const void *kernel);          // blocks filling the GPU at 
 // plan the LDS staging of a launch's tables (bytes = 0 when they do not fit)
 LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes);
 int launch_blocks(int fill, const KArgs &a, int block = 256);   // capped by max_blocks and the pixel count
+int max_lds_bytes();                // LDS bytes a block may use (FSG_LDS_BYTES caps it)
 
 // FSG_PREC_FP32X kernels (fsg_fp32x.hip)
 constexpr size_t XS_BYTES = 12;

@@ -209,7 +209,7 @@ int launch_blocks(int fill, const KArgs &a, int block) {
     return blocks;
 }
 
-static int max_lds_bytes() {
+int max_lds_bytes() {
     static int cached = -1;
     if (cached < 0) {
         int dev = 0, v = 0;
