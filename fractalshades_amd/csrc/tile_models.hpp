@@ -250,7 +250,7 @@ struct HoloXr {
         if (s.n >= a.max_iter) return 0;
         if (a.calc_dz) {
             const xz nx = s.nul ? DZz(t, 0) : (s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
-            if (lt(abs2(as_z(s.h) + nx), a.eps2)) return 2;
+            if (abs2_lt(as_z(s.h) + nx, a.eps2)) return 2;
         }
         const cplx ZZ = s.zs + t.Zn[s.w];
         s.orb.record(s.n, a.backshift, ZZ.re, ZZ.im);

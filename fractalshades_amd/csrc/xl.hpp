@@ -184,6 +184,12 @@ FSG_DEV xz operator*(xz a, cplx b) {
 }
 FSG_DEV xz muls(double s, xz a) { xz r; r.re = s * a.re; r.im = s * a.im; r.e = a.e; return r; }   // s != 0
 FSG_DEV xlf abs2(xz a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
+// |a|^2 < b for a binary64 b > 0: the mantissa sum rounds as abs2's, and
+// scaling it by 2^(2e) is exact unless the result leaves the normal range,
+// where it lies far on one side of b whichever way it rounds
+FSG_DEV bool abs2_lt(xz a, double b) {
+    return ldexp(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))) < b;
+}
 
 // Lazily normalised reals for the burning-ship step (the same argument as
 // xz: mantissas of the operands start in [1, 2), one iteration's products
