@@ -26,7 +26,7 @@ EXPORTS = (
     "fsg_postproc_device", "fsg_server_submit_layers", "fsg_ref_orbit",
     "fsg_ball_period", "fsg_newton_eval",
 )
-FSG_E_RANGE = -5
+FSG_E_INVALID, FSG_E_HIP, FSG_E_NOMEM, FSG_E_UNSUPPORTED, FSG_E_RANGE = -1, -2, -3, -4, -5
 FSG_PENDING = 2
 
 

@@ -37,3 +37,16 @@ def test_desc_layout_matches_header():
     assert d.calc_orbit.offset == 212 and d.backshift.offset == 216
     assert d.deriv_mant.offset == 224 and d.deriv_exp.offset == 232
     assert native.ctypes.sizeof(d) == 240
+
+
+def test_argument_errors_without_gpu():
+    """Argument checks that return before any device call: a null context
+    or a negative grid width is FSG_E_INVALID with a message."""
+    import ctypes
+    lib = native.load()
+    st = native.Stats()
+    rc = lib.fsg_tile_grid(None, None, 0, 8, None, None, None, None, None, ctypes.byref(st))
+    assert rc == native.FSG_E_INVALID
+    assert b"null context" in lib.fsg_last_error()
+    rc = lib.fsg_tile_grid_device(None, None, 0, 0, None, None, None, None, None, None)
+    assert rc == native.FSG_E_INVALID
