@@ -504,8 +504,25 @@ int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
     }
     if (c->d.precision == FSG_PREC_FP32X && !f32_supported(cfg_of(c)))
         return cleanup(fail(FSG_E_UNSUPPORTED, "no FP32X kernel for this model"));
-    // binary32 copies of the holomorphic tables (the burning-ship form
-    // converts its real tables as it reads them)
+    // binary32 copies of the burning-ship orbit and Jacobian paths
+    if (c->d.precision == FSG_PREC_FP32X && c->d.kind == FSG_KIND_BS) {
+        hipError_t xe;
+        void *p = dalloc<char>((size_t)L * 2 * XSR_BYTES, xe);
+        c->owned.push_back(p); c->tx.ZRx = p;
+        if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        for (int q = 0; q < 4; q++) {
+            if (!c->bs[q]) continue;
+            p = dalloc<char>((size_t)L * XSR_BYTES, xe);
+            c->owned.push_back(p); c->tx.BSx[q] = p;
+            if (xe != hipSuccess) return cleanup(fail(FSG_E_NOMEM, "hipMalloc FP32X tables"));
+        }
+        KArgs ka;
+        fill_args(c, ka);
+        hipError_t be = build_xsr_tables(ka.t, c->tx, c->stream);
+        if (be != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            return cleanup(fail(FSG_E_HIP, "FP32X table build failed"));
+    }
+    // binary32 copies of the holomorphic tables
     if (c->d.precision == FSG_PREC_FP32X && c->d.kind != FSG_KIND_BS) {
         hipError_t xe;
         void *p;

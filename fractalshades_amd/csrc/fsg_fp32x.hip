@@ -385,15 +385,52 @@ __device__ __forceinline__ double ddx(xzs X, xzs x) {
     return Xx.m <= 0.f ? -1. : 1.;
 }
 
+// per-reference burning-ship tables in the xsr format, built once per
+// context: the orbit (the Xrange register below 1e-300) and the Jacobian paths
+__global__ void k_xsr_orbit(const cplx *Zn, int32_t L, const int32_t *xr_idx, const double *xr_m,
+                            const int32_t *xr_e, int32_t n_xr, xsr *out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    int32_t lo = 0, hi = n_xr;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (xr_idx[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    if (lo < n_xr && xr_idx[lo] == i) {
+        out[2 * i] = xsr_from(xr_m[2 * lo], xr_e[2 * lo]);
+        out[2 * i + 1] = xsr_from(xr_m[2 * lo + 1], xr_e[2 * lo + 1]);
+    } else {
+        out[2 * i] = xsr_from(Zn[i].re, 0);
+        out[2 * i + 1] = xsr_from(Zn[i].im, 0);
+    }
+}
+
+__global__ void k_xsr_path(const double *m, const int32_t *e, int32_t n, xsr *out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = xsr_from(m[i], e ? e[i] : 0);
+}
+
+hipError_t build_xsr_tables(const Tables &t, TablesX &x, hipStream_t stream) {
+    const int tb = 256;
+    hipLaunchKernelGGL(k_xsr_orbit, dim3((t.L + tb - 1) / tb), dim3(tb), 0, stream, t.Zn, t.L,
+                       t.xr_idx, t.xr_m, t.xr_e, t.n_xr, (xsr *)x.ZRx);
+    for (int q = 0; q < 4; q++)
+        if (t.bs[q] && x.BSx[q])
+            hipLaunchKernelGGL(k_xsr_path, dim3((t.L + tb - 1) / tb), dim3(tb), 0, stream,
+                               t.bs[q], t.bs_e[q], t.L, (xsr *)x.BSx[q]);
+    return hipGetLastError();
+}
+
 template <int FL>
 struct BsF32 {
     struct State { xsr x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; };
 
-    __device__ static xsr PATH(const Tables &t, int q, int32_t i) { return xsr_from(t.bs[q][i], t.bs_e[q][i]); }
-    __device__ static void ZR(const Tables &t, int32_t i, xsr &X, xsr &Y) {
-        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
-        if (j >= 0) { X = xsr_from(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xsr_from(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
-        else { const cplx R = t.Zn[i]; X = xsr_from(R.re, 0); Y = xsr_from(R.im, 0); }
+    // the tables hold xsr_from of the binary64 values (build_xsr_tables)
+    __device__ static xsr PATH(const KArgs &a, int q, int32_t i) { return static_cast<const xsr *>(a.tx.BSx[q])[i]; }
+    __device__ static void ZR(const KArgs &a, int32_t i, xsr &X, xsr &Y) {
+        const xsr *z = static_cast<const xsr *>(a.tx.ZRx);
+        X = z[2 * i]; Y = z[2 * i + 1];
     }
 
     __device__ static void init(State &s, const KArgs &a, double px, double py) {
@@ -431,13 +468,13 @@ struct BsF32 {
         }
         s.n += 1; k.plain++;
         xsr X, Y;
-        ZR(t, s.w, X, Y);
+        ZR(a, s.w, X, Y);
         // one iteration on lazily normalised values, renormalised once
         const xzs x = as_zs(s.x), y = as_zs(s.y), Xz = as_zs(X), Yz = as_zs(Y);
         if (a.calc_dc) {
             const xzs z0 = xzs{0.f, XS_ZERO_E};
             xzs Rd[4] = {z0, z0, z0, z0};
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = as_zs(PATH(t, q, s.w));
+            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = as_zs(PATH(a, q, s.w));
             xzs P[4] = {as_zs(s.P[0]), as_zs(s.P[1]), as_zs(s.P[2]), as_zs(s.P[3])};
             bs_hess<FL>(x, y, P, Xz, Yz, Rd);
             for (int q = 0; q < 4; q++) s.P[q] = xsr_of(P[q]);
@@ -456,7 +493,7 @@ struct BsF32 {
         if (s.w >= t.div_iter - 1) {
             s.xs = XX; s.ys = YY;
             s.x = xsr_from(XX, 0); s.y = xsr_from(YY, 0);
-            if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
+            if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(a, q, s.w);
             s.w = 0;
             k.reb++;
             return -1;
@@ -464,12 +501,12 @@ struct BsF32 {
         s.dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
         if (s.dyn) {
             xsr X1, Y1;
-            ZR(t, s.w, X1, Y1);
+            ZR(a, s.w, X1, Y1);
             const xsr XXx = s.x + X1, YYx = s.y + Y1;
             if (xsr_le(XXx * XXx + YYx * YYx, s.x * s.x + s.y * s.y)) {
                 s.x = XXx; s.y = YYx;
                 s.xs = xsr_to_d(XXx); s.ys = xsr_to_d(YYx);
-                if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
+                if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(a, q, s.w);
                 s.w = 0;
                 k.reb++;
             }
@@ -482,7 +519,7 @@ struct BsF32 {
         const cplx R = a.t.Zn[s.w];
         a.Z[i] = s.xs + R.re; a.Z[n + i] = s.ys + R.im;
         if (a.calc_dc)
-            for (int q = 0; q < 4; q++) a.Z[(2 + q) * n + i] = xsr_to_d(s.P[q] + PATH(a.t, q, s.w));
+            for (int q = 0; q < 4; q++) a.Z[(2 + q) * n + i] = xsr_to_d(s.P[q] + PATH(a, q, s.w));
         a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
     }
 };

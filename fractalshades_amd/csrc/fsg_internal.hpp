@@ -33,6 +33,10 @@ struct TablesX {
     const void *DCx;           // [L] dZn/dc
     const void *DZx;           // [L+1] dZn/dz
     const void *Ax, *Bx;       // [bla_len] BLA coefficients
+    // burning ship: orbit as (x, y) pairs and the 4 Jacobian paths, each a
+    // binary32 mantissa with an int32 exponent (8 bytes)
+    const void *ZRx;           // [L] pairs
+    const void *BSx[4];        // [L]
 };
 
 // Layout of the reference tables staged in LDS for one launch (byte
@@ -103,6 +107,8 @@ int max_lds_bytes();                // LDS bytes a block may use (FSG_LDS_BYTES 
 // FSG_PREC_FP32X kernels (fsg_fp32x.hip)
 constexpr size_t XS_BYTES = 12;
 hipError_t build_xs_tables(const Tables &t, int64_t bla_len, TablesX &x, hipStream_t stream);
+hipError_t build_xsr_tables(const Tables &t, TablesX &x, hipStream_t stream);   // burning ship
+constexpr size_t XSR_BYTES = 8;
 hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream);
 bool f32_supported(const LaunchCfg &cfg);
 
