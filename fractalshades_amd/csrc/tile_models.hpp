@@ -438,7 +438,9 @@ struct BsXr {
         if (a.calc_dc) {
             const xzf z0 = as_zf(xlf_zero());
             xzf Rd[4] = {z0, z0, z0, z0};
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = as_zf(PATH(t, q, s.w));
+            // the host's Xrange path records keep |mantissa| within 2^+-100: used
+            // unnormalised, as HoloXr's DCz
+            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = zf_mk(t.bs[q][s.w], t.bs_e[q][s.w]);
             xzf P[4] = {as_zf(s.P[0]), as_zf(s.P[1]), as_zf(s.P[2]), as_zf(s.P[3])};
             bs_hess<FL>(x, y, P, Xz, Yz, Rd);
             for (int q = 0; q < 4; q++) s.P[q] = xlf_of(P[q]);
