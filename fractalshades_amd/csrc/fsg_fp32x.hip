@@ -554,7 +554,7 @@ static tile_fn select_f32(const LaunchCfg &c) {
 // (the binary64 k_tile_lds<Model, true> for this form's tables): binary64
 // orbit and BLA radii for the tests, xs orbit, derivative paths (the
 // wrapped dZndz value at [K]) and BLA coefficients.
-template <class Model>
+template <class Model, bool FULL>
 __global__ void __launch_bounds__(1024) k_tile_lds_x(const KArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsPlanX &p = a.ldsx;
@@ -562,13 +562,15 @@ __global__ void __launch_bounds__(1024) k_tile_lds_x(const KArgs a) {
     Tables &t = al.t;
     TablesX &x = al.tx;
     const int64_t K = p.K, KB = p.KB;
-    t.Zn = stage<true>(lds, p.zn, a.t.Zn, K);
-    t.r = stage<true>(lds, p.r, a.t.r, KB);
-    x.Zx = stage<true>(lds, p.zx, static_cast<const xs *>(a.tx.Zx), K);
-    x.DCx = stage<true>(lds, p.dcx, static_cast<const xs *>(a.tx.DCx), K);
-    x.Ax = stage<true>(lds, p.ax, static_cast<const xs *>(a.tx.Ax), KB);
-    x.Bx = stage<true>(lds, p.bx, static_cast<const xs *>(a.tx.Bx), KB);
-    if (a.tx.DZx) {
+    t.Zn = stage<FULL>(lds, p.zn, a.t.Zn, K);
+    t.r = stage<FULL>(lds, p.r, a.t.r, KB);
+    x.Zx = stage<FULL>(lds, p.zx, static_cast<const xs *>(a.tx.Zx), K);
+    x.DCx = stage<FULL>(lds, p.dcx, static_cast<const xs *>(a.tx.DCx), K);
+    x.Ax = stage<FULL>(lds, p.ax, static_cast<const xs *>(a.tx.Ax), KB);
+    x.Bx = stage<FULL>(lds, p.bx, static_cast<const xs *>(a.tx.Bx), KB);
+    if (!FULL && p.dzx < 0) {
+        // read from global memory
+    } else if (a.tx.DZx) {
         const xs *src = static_cast<const xs *>(a.tx.DZx);
         xs *dz = reinterpret_cast<xs *>(lds + p.dzx);
         for (int64_t i = threadIdx.x; i < K; i += blockDim.x) dz[i] = src[i];
@@ -597,17 +599,26 @@ static LdsPlanX plan_lds_x(const Tables &t, const TablesX &x, int max_bytes) {
     if (x.DZx) p.dzx = take((int64_t)XS_BYTES * (K + 1));
     if (KB > 0) { p.r = take(8 * KB); p.ax = take((int64_t)XS_BYTES * KB); p.bx = take((int64_t)XS_BYTES * KB); }
     p.K = (int32_t)K; p.KB = (int32_t)KB;
-    p.bytes = (off <= max_bytes && (!t.bla_on || KB > 0)) ? (int32_t)off : 0;
+    if (off <= max_bytes && (!t.bla_on || KB > 0)) { p.bytes = (int32_t)off; return p; }
+    // partial: the xs orbit and the BLA tables, the rest from global memory
+    std::memset(&p, 0xff, sizeof(p));
+    off = 0;
+    p.zx = take((int64_t)XS_BYTES * K);
+    if (KB > 0) { p.r = take(8 * KB); p.ax = take((int64_t)XS_BYTES * KB); p.bx = take((int64_t)XS_BYTES * KB); }
+    p.K = (int32_t)K; p.KB = (int32_t)KB;
+    p.bytes = (off <= max_bytes) ? -(int32_t)off : 0;    // negative: a partial plan
     return p;
 }
 
-template <class Model> static tile_fn fn_lds_x() { return k_tile_lds_x<Model>; }
+template <class Model> static tile_fn fn_lds_x(bool full) {
+    return full ? k_tile_lds_x<Model, true> : k_tile_lds_x<Model, false>;
+}
 
-static tile_fn select_f32_lds(const LaunchCfg &c) {
-    if (c.kind == KIND_M2) return fn_lds_x<HoloF32<KIND_M2, 2>>();
+static tile_fn select_f32_lds(const LaunchCfg &c, bool full) {
+    if (c.kind == KIND_M2) return fn_lds_x<HoloF32<KIND_M2, 2>>(full);
     if (c.kind == KIND_MN) {
         switch (c.degree) {
-#define FSG_MN(D) case D: return fn_lds_x<HoloF32<KIND_MN, D>>();
+#define FSG_MN(D) case D: return fn_lds_x<HoloF32<KIND_MN, D>>(full);
             FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
 #undef FSG_MN
             default: return nullptr;
@@ -623,7 +634,9 @@ hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t str
     if (cfg.kind != KIND_BS && !cfg.orbit && !no_lds) {
         KArgs al = a;
         al.ldsx = plan_lds_x(a.t, a.tx, max_lds_bytes());
-        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg) : nullptr;
+        const bool full = al.ldsx.bytes > 0;
+        if (al.ldsx.bytes < 0) al.ldsx.bytes = -al.ldsx.bytes;
+        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg, full) : nullptr;
         int per_cu = 0, dev = 0, ncu = 0;
         if (fl && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
