@@ -271,6 +271,9 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
         }
     }
     if (KB > 0 && group({8 * KB, 8 * bla_dim * KB})) { p.r = take(8 * KB); p.M = take(8 * bla_dim * KB); }
+    // else the radii alone: the stage bisection reads them in a dependent
+    // chain, the coefficients once per jump
+    else if (KB > 0 && group({8 * KB})) p.r = take(8 * KB);
     p.K = (int32_t)K; p.KB = (int32_t)KB; p.bla_dim = bla_dim;
     p.bytes = (int32_t)off;
     return p;
