@@ -599,14 +599,15 @@ static LdsPlanX plan_lds_x(const Tables &t, const TablesX &x, int max_bytes) {
     if (x.DZx) p.dzx = take((int64_t)XS_BYTES * (K + 1));
     if (KB > 0) { p.r = take(8 * KB); p.ax = take((int64_t)XS_BYTES * KB); p.bx = take((int64_t)XS_BYTES * KB); }
     p.K = (int32_t)K; p.KB = (int32_t)KB;
-    if (off <= max_bytes && (!t.bla_on || KB > 0)) { p.bytes = (int32_t)off; return p; }
+    if (off <= max_bytes && (!t.bla_on || KB > 0)) { p.bytes = (int32_t)off; p.full = 1; return p; }
     // partial: the xs orbit and the BLA tables, the rest from global memory
     std::memset(&p, 0xff, sizeof(p));
     off = 0;
     p.zx = take((int64_t)XS_BYTES * K);
     if (KB > 0) { p.r = take(8 * KB); p.ax = take((int64_t)XS_BYTES * KB); p.bx = take((int64_t)XS_BYTES * KB); }
     p.K = (int32_t)K; p.KB = (int32_t)KB;
-    p.bytes = (off <= max_bytes) ? -(int32_t)off : 0;    // negative: a partial plan
+    p.bytes = (off <= max_bytes) ? (int32_t)off : 0;
+    p.full = 0;
     return p;
 }
 
@@ -634,9 +635,7 @@ hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t str
     if (cfg.kind != KIND_BS && !cfg.orbit && !no_lds) {
         KArgs al = a;
         al.ldsx = plan_lds_x(a.t, a.tx, max_lds_bytes());
-        const bool full = al.ldsx.bytes > 0;
-        if (al.ldsx.bytes < 0) al.ldsx.bytes = -al.ldsx.bytes;
-        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg, full) : nullptr;
+        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg, al.ldsx.full != 0) : nullptr;
         int per_cu = 0, dev = 0, ncu = 0;
         if (fl && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&

@@ -48,10 +48,11 @@ struct LdsPlan {
     int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, r, M;
 };
 
-// The binary32 form's tables staged in LDS (byte offsets, -1 = absent):
-// orbit (binary64, for the escape tests), BLA radii, and the xs orbit,
-// derivative paths and BLA coefficients; all of them or none.
-struct LdsPlanX { int32_t K, KB, bytes, zn, r, zx, dcx, dzx, ax, bx; };
+// The binary32 form's tables staged in LDS (byte offsets, -1 = read from
+// global memory): orbit (binary64, for the escape tests), BLA radii, and the
+// xs orbit, derivative paths and BLA coefficients.  full: all of them; else
+// the xs orbit and the BLA tables only.
+struct LdsPlanX { int32_t K, KB, bytes, full, zn, r, zx, dcx, dzx, ax, bx; };
 
 // One launch of the tile kernel.
 struct KArgs {
