@@ -527,6 +527,52 @@ struct BsF32 {
 typedef void (*tile_fn)(const KArgs);
 template <class Model> static tile_fn fn_of() { return k_tile<Model>; }
 
+// The burning-ship binary32 loop with the tables its step reads most staged
+// in LDS: the xsr orbit pairs, the binary64 orbit of the escape tests and
+// the BLA radii and coefficients (the Jacobian paths stay in global memory).
+// Every staged pointer is an LDS address, every other one global.
+template <int FL>
+__global__ void __launch_bounds__(1024) k_tile_lds_bsx(const KArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsPlanX &p = a.ldsx;
+    KArgs al = a;
+    const int64_t K = p.K, KB = p.KB;
+    al.t.Zn = stage<true>(lds, p.zn, a.t.Zn, K);
+    al.t.r = stage<true>(lds, p.r, a.t.r, KB);
+    al.t.M = stage<true>(lds, p.ax, a.t.M, KB * 8);     // the plan's ax slot: binary64 coefficients
+    al.tx.ZRx = stage<true>(lds, p.zx, static_cast<const xsr *>(a.tx.ZRx), 2 * K);
+    __syncthreads();
+    tile_loop<BsF32<FL>>(al);
+}
+
+static LdsPlanX plan_lds_bsx(const Tables &t, int max_bytes) {
+    LdsPlanX p;
+    std::memset(&p, 0xff, sizeof(p));
+    int64_t K = std::min<int64_t>(t.L, std::min<int64_t>((int64_t)t.div_iter + 1, t.order));
+    K = std::max<int64_t>(K, std::min<int64_t>(t.L, 2));
+    const int64_t bla_len = 2 * (int64_t)(t.L >> 3);
+    const int64_t KB = t.bla_on ? std::min<int64_t>(bla_len, K / 4 + 2) : 0;
+    int64_t off = 0;
+    const auto take = [&](int64_t bytes) { int64_t o = off; off += (bytes + 15) & ~(int64_t)15; return (int32_t)o; };
+    p.zx = take((int64_t)XSR_BYTES * 2 * K);
+    p.zn = take(16 * K);
+    if (KB > 0) { p.r = take(8 * KB); p.ax = take(64 * KB); }
+    p.K = (int32_t)K; p.KB = (int32_t)KB; p.full = 0;
+    p.bytes = (off <= max_bytes && (!t.bla_on || KB > 0)) ? (int32_t)off : 0;
+    return p;
+}
+
+static tile_fn select_bsx_lds(const LaunchCfg &c) {
+    switch (c.flavor) {
+        case 1: return k_tile_lds_bsx<1>;
+        case 2: return k_tile_lds_bsx<2>;
+        case 3: return k_tile_lds_bsx<3>;
+        case 4: return k_tile_lds_bsx<4>;
+        case 5: return k_tile_lds_bsx<5>;
+        default: return nullptr;
+    }
+}
+
 static tile_fn select_f32(const LaunchCfg &c) {
     if (c.kind == KIND_M2) return fn_of<HoloF32<KIND_M2, 2>>();
     if (c.kind == KIND_BS) {
@@ -632,10 +678,12 @@ hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t str
     // FSG_TABLES=global keeps the tables in global memory (for comparisons)
     const char *tables = std::getenv("FSG_TABLES");
     const bool no_lds = tables && !std::strcmp(tables, "global");
-    if (cfg.kind != KIND_BS && !cfg.orbit && !no_lds) {
+    if (!cfg.orbit && !no_lds && (cfg.kind != KIND_BS || a.tx.ZRx)) {
         KArgs al = a;
-        al.ldsx = plan_lds_x(a.t, a.tx, max_lds_bytes());
-        tile_fn fl = al.ldsx.bytes > 0 ? select_f32_lds(cfg, al.ldsx.full != 0) : nullptr;
+        const bool bs = cfg.kind == KIND_BS;
+        al.ldsx = bs ? plan_lds_bsx(a.t, max_lds_bytes()) : plan_lds_x(a.t, a.tx, max_lds_bytes());
+        tile_fn fl = al.ldsx.bytes <= 0 ? nullptr
+                     : bs ? select_bsx_lds(cfg) : select_f32_lds(cfg, al.ldsx.full != 0);
         int per_cu = 0, dev = 0, ncu = 0;
         if (fl && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&

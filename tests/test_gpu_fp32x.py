@@ -109,21 +109,28 @@ def test_fp32x_burning_ship_flavours(flavor):
     assert rep["nu_rel_err_median"] < 1e-5 and rep["nu_rel_err_p99"] < 1e-3
 
 
-@pytest.mark.parametrize("case", ["M2_E20", "M2_divref5", "Mn3", "C2", "C2_2000"])
+@pytest.mark.parametrize("case", ["M2_E20", "M2_divref5", "Mn3", "C2", "C2_2000", "BS_ex14", "C3_bs"])
 def test_fp32x_lds_matches_global(case, monkeypatch):
     """The binary32 loop with its tables staged in LDS (k_tile_lds_x, the
     default when they fit; at 1e-2000 only the xs orbit and the BLA tables
-    fit and the rest stays in global memory) gives the same bits as with
-    the tables in global memory (FSG_TABLES=global): only where the tables
-    are read from changes."""
+    fit and the rest stays in global memory; k_tile_lds_bsx for the burning
+    ship) gives the same bits as with the tables in global memory
+    (FSG_TABLES=global): only where the tables are read from changes."""
     from gpu_util import assert_same
     if case == "Mn3":
         f = frames.mn_frame(3, nx=96)
         args, c_pix = frames.frame_inputs(f)
-    elif case in ("C2", "C2_2000"):
+    elif case in ("C2", "C2_2000", "C3_bs"):
+        try:
+            configs.center_for(case, compute=False)
+        except KeyError:
+            pytest.skip(f"no cached seeded centre for {case}")
         f = configs.build(configs.WORKLOADS[case], nx=512, ny=512)
         args = f._calc_data["bench"]["cycle_indep_args"]
         c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
+    elif case == "BS_ex14":
+        f = frames.make(case, nx=96, max_iter=6000)
+        args, c_pix = frames.frame_inputs(f)
     else:
         f = frames.make(case, nx=96)
         args, c_pix = frames.frame_inputs(f)
