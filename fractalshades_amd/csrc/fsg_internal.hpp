@@ -91,6 +91,7 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
 typedef void (*tile_fn)(const KArgs);
 tile_fn select_tile_orbit(const LaunchCfg &cfg);
 tile_fn select_tile_lds(const LaunchCfg &cfg, bool full);
+int tile_lds_block(const LaunchCfg &cfg);        // lanes per block of select_tile_lds's kernel
 // true when the plan stages every table the launch reads
 bool lds_plan_full(const Tables &t, const LdsPlan &p);
 // radii_only: M already holds the coefficients (shared with another

@@ -295,12 +295,13 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
         KArgs al = a;
         al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());
         tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg, lds_plan_full(a.t, al.lds)) : nullptr;
+        const int block = tile_lds_block(cfg);
         int per_cu = 0, dev = 0, ncu = 0;
         if (f && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), 1024,
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), block,
                                                          (size_t)al.lds.bytes) == hipSuccess && per_cu > 0) {
-            hipLaunchKernelGGL(f, dim3(launch_blocks(ncu * per_cu, al, 1024)), dim3(1024), (size_t)al.lds.bytes,
+            hipLaunchKernelGGL(f, dim3(launch_blocks(ncu * per_cu, al, block)), dim3(block), (size_t)al.lds.bytes,
                                stream, al);
             return hipGetLastError();
         }

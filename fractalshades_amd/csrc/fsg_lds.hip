@@ -8,8 +8,23 @@ namespace fsg {
 template <class Model> static tile_fn fn_lds(bool full) {
     return full ? k_tile_lds<Model, true> : k_tile_lds<Model, false>;
 }
+// z^d + c of degree 6..8 with extended-range deltas needs more than the 128
+// VGPRs a 1024-lane block allows (it spilled): 512-lane blocks
+template <class Model> static tile_fn fn_lds512(bool full) {
+    return full ? k_tile_lds<Model, true, 512> : k_tile_lds<Model, false, 512>;
+}
+
+int tile_lds_block(const LaunchCfg &c) { return (c.kind == KIND_MN && c.xr && c.degree >= 6) ? 512 : 1024; }
 
 tile_fn select_tile_lds(const LaunchCfg &c, bool full) {
+    if (tile_lds_block(c) == 512) {
+        switch (c.degree) {
+            case 6: return fn_lds512<HoloXr<KIND_MN, 6>>(full);
+            case 7: return fn_lds512<HoloXr<KIND_MN, 7>>(full);
+            case 8: return fn_lds512<HoloXr<KIND_MN, 8>>(full);
+            default: return nullptr;
+        }
+    }
     if (c.kind == KIND_M2) return c.xr ? fn_lds<HoloXr<KIND_M2, 2>>(full) : fn_lds<HoloStd<KIND_M2, 2>>(full);
     if (c.kind == KIND_MN) {
         switch (c.degree) {

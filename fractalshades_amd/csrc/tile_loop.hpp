@@ -178,8 +178,8 @@ __device__ __forceinline__ const T *stage(unsigned char *lds, int32_t off, const
     return dst;
 }
 
-template <class Model, bool FULL>
-__global__ void __launch_bounds__(1024) k_tile_lds(const KArgs a) {
+template <class Model, bool FULL, int BLOCK = 1024>
+__global__ void __launch_bounds__(BLOCK) k_tile_lds(const KArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsPlan &p = a.lds;
     KArgs al = a;
