@@ -63,7 +63,9 @@ def mn_frame(d, nx=48, xr=False, **calc_over):
     return f
 
 
-def make(name, nx=48, model_kw=None, **calc_over):
+def make(name, nx=48, model_kw=None, no_newton=False, **calc_over):
+    """The frame `name` at nx pixels per row; no_newton: reference orbit from
+    the image centre, not a cycle (the reference's settings.no_newton)."""
     model, zoom_kw, calc_kw = FRAMES[name]
     if model == "BS" and model_kw is None:
         model_kw = {"flavor": FRAMES[name + "_flavor"]}
@@ -74,6 +76,7 @@ def make(name, nx=48, model_kw=None, **calc_over):
     else:
         f = fsa.Perturbation_burning_ship("/tmp/fsa_test", **(model_kw or {}))
     f.zoom(nx=nx, xy_ratio=1.0, theta_deg=0., **zoom_kw)
+    f.no_newton = no_newton
     kw = dict(calc_name="t", subset=None, M_divergence=1.e3, epsilon_stationnary=1.e-3,
               BLA_eps=1.e-6)
     if model == "BS":

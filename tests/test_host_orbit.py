@@ -308,3 +308,83 @@ def test_frame_reference_equals_mpmath(name):
         assert abs(pm["ref_point"] - pn["ref_point"]) <= abs(pm["ref_point"]) * mpmath.ldexp(1, -prec + 4)
     np.testing.assert_array_equal(Zm.view(np.uint64), Zn.view(np.uint64))
     assert pm["xr"] == pn["xr"]
+
+
+# ---- the remaining known answers of tests/test_FP_loop.py ----------------------
+def test_newton_nucleus_kat():
+    """tests/test_FP_loop.py:231-272: the divisor-removal Newton of z^2 + c at
+    order 4252 and 11622 bits from the reference's start point converges to
+    its ~3500-digit nucleus (mpmath.almosteq at precision - 10 bits, as the
+    reference compares).  The orbit and derivative come from the fixed-point
+    loop (fsg_newton_eval), the step is formed in mpmath."""
+    k = KATS["newton"]
+    prec = k["precision_bits"]
+    eps = mpmath.mpf(2) ** -prec
+    eps_pix = mpmath.mpf(2) ** -k["pix_bits"]
+    ok, val = fo.find_nucleus_m2(_c(k["x_start"], k["y_start"], prec), k["order"], eps_pix, prec,
+                                 max_newton=k["max_newton"], eps_cv=eps)
+    assert ok
+    with mpmath.workprec(prec - 10):
+        assert mpmath.almosteq(mpmath.mpf(k["x_nucleus"]), val.real)
+        assert mpmath.almosteq(mpmath.mpf(k["y_nucleus"]), val.imag)
+    # the start point itself is off by ~1e-82: the test is not vacuous
+    with mpmath.workprec(prec):
+        assert abs(mpmath.mpf(k["x_start"]) - mpmath.mpf(k["x_nucleus"])) > mpmath.mpf("1e-90")
+
+
+@pytest.mark.parametrize("backend", ["native", "mpmath"])
+def test_bs_path_kat(backend):
+    """tests/test_FP_loop.py:278-309: 10 burning-ship steps from c = 0.1 + 0.1i
+    equal the binary64 recurrence x' = x^2 - y^2 + a, y' = 2|x y| - b.  The
+    reference compares bit for bit against MPFR at 53 bits (one rounding per
+    operation); the orbit loops here round each exact iterate once, so the
+    two may differ by an ulp of an iterate: compared within 4 ulp."""
+    k = KATS["bs_path"]
+    a, b = k["x"], k["y"]
+    loop = fo if backend == "native" else ml
+    # npts entries, iterates 1 .. npts - 1 (FP_loop's max_iter = npts - 1)
+    n, Zn, _ = loop.bs_orbit(_c(repr(a), repr(b), 300), 1, k["npts"], k["M"], False, 300)
+    assert n == k["npts"]
+    z = 0j
+    for i in range(k["npts"]):
+        assert abs(Zn[i] - z) <= 4 * np.finfo(float).eps * abs(z), (i, Zn[i], z)
+        x, y = z.real, z.imag
+        z = complex(x * x - y * y + a, 2 * abs(x * y) - b)
+
+
+@pytest.mark.parametrize("backend", ["native", "mpmath"])
+def test_bs_ball_order_kat(backend):
+    """tests/test_FP_loop.py:311-341: burning-ship ball-method period 3 at
+    two points (px 0.01, 100 steps, M 1000, 53 bits)."""
+    k = KATS["bs_order"]
+    loop = fo if backend == "native" else ml
+    prec = k["precision_bits"]
+    for (x, y, px), want in zip(k["points"], k["orders"]):
+        order = loop.bs_ball_method(_c(x, y, prec), mpmath.mpf(px), 1, k["maxiter"],
+                                    k["M_divergence"], prec)
+        assert order == want
+
+
+@pytest.mark.parametrize("backend", ["native", "mpmath"])
+def test_bs_newton_kat(backend):
+    """tests/test_FP_loop.py:343-382: the burning ship's 2x2 Newton at order 3
+    from the two start points.  The reference only checks that it runs; here
+    the result must also be a period-3 nucleus (x_3, y_3 = 0 to 1e-12 by
+    direct iteration) within the seed radius of the start point."""
+    from fractalshades_amd.mpmath_loop import bs_iterate
+    k = KATS["bs_newton"]
+    loop = fo if backend == "native" else ml
+    for case in k["cases"]:
+        prec = case["prec"]
+        start = _c(case["x"], case["y"], prec)
+        ok, val = loop.bs_find_any_nucleus(start, case["order"], 1, mpmath.mpf(case["px"]), prec,
+                                           max_newton=k["max_newton"],
+                                           eps_cv=mpmath.mpf(2) ** -prec)
+        assert ok, case
+        with mpmath.workprec(200):
+            a, b = mpmath.mpf(val.real), mpmath.mpf(val.imag)
+            x = y = mpmath.mpf(0)
+            for _ in range(case["order"]):
+                x, y = bs_iterate(1, x, y, a, b)
+            assert mpmath.hypot(x, y) < 1e-12
+            assert abs(mpmath.mpc(a, b) - start) < 10 * mpmath.mpf(case["px"])
