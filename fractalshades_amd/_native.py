@@ -24,8 +24,10 @@ EXPORTS = (
     "fsg_server_create", "fsg_server_destroy", "fsg_server_footprint", "fsg_server_clock",
     "fsg_server_submit", "fsg_server_wait", "fsg_host_alloc", "fsg_host_free",
     "fsg_postproc_device", "fsg_server_submit_layers", "fsg_ref_orbit",
-    "fsg_ball_period", "fsg_newton_eval",
+    "fsg_ball_period", "fsg_newton_eval", "fsg_tile_grid_noalloc",
 )
+# device staging of fsg_tile_grid_noalloc (FSG_NOALLOC_SCRATCH_BYTES)
+FSG_NOALLOC_SCRATCH_BYTES = 32 << 20
 FSG_E_INVALID, FSG_E_HIP, FSG_E_NOMEM, FSG_E_UNSUPPORTED, FSG_E_RANGE = -1, -2, -3, -4, -5
 FSG_PENDING = 2
 
@@ -168,6 +170,8 @@ def load(path=None):
             "fsg_tile_device": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "fsg_tile_grid": ([vp, vp, i64, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "fsg_tile_grid_device": ([vp, vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "fsg_tile_grid_noalloc": ([vp, vp, i64, i64, vp, vp, vp, vp, vp, ctypes.POINTER(Stats)],
+                                      ctypes.c_int),
             "fsg_work_bytes": ([], ctypes.c_size_t),
             "fsg_work_stats": ([vp, ctypes.POINTER(Stats)], ctypes.c_int),
             "fsg_ctx_read_table": ([vp, ctypes.c_int, vp, ctypes.c_size_t], ctypes.c_int),
@@ -262,9 +266,11 @@ class GpuContext:
         except Exception:
             pass
 
-    def tile(self, c_pix, Z, U, stop_reason, stop_iter, interrupted=None, grid_w=0):
+    def tile(self, c_pix, Z, U, stop_reason, stop_iter, interrupted=None, grid_w=0, noalloc=False):
         """Run one tile in place; returns (rc, stats dict).  grid_w > 0: c_pix
-        holds rows of grid_w pixels, computed in 8x8 blocks (same results)."""
+        holds rows of grid_w pixels, computed in 8x8 blocks (same results).
+        noalloc: through fsg_tile_grid_noalloc (static device staging, no
+        allocation; at most FSG_NOALLOC_SCRATCH_BYTES of staging)."""
         for name, arr in (("c_pix", c_pix), ("Z", Z), ("U", U),
                           ("stop_reason", stop_reason), ("stop_iter", stop_iter)):
             if not arr.flags.c_contiguous:
@@ -281,7 +287,8 @@ class GpuContext:
         intr = None
         if interrupted is not None:
             intr = interrupted.ctypes.data
-        rc = check(load().fsg_tile_grid(
+        entry = load().fsg_tile_grid_noalloc if noalloc else load().fsg_tile_grid
+        rc = check(entry(
             self._h, _ptr(c_pix), n, int(grid_w), _ptr(Z), _ptr(U), _ptr(stop_reason),
             _ptr(stop_iter), intr, ctypes.byref(st)))
         return rc, st.as_dict()

@@ -382,6 +382,86 @@ int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, doubl
 size_t work_words() { return WORK_WORDS; }
 }  // namespace fsg
 
+namespace {
+
+// ---- device staging of the host-buffer entry points ----
+// bytes of staging for n pixels: c_pix, Z, U, stop_iter, stop_reason
+size_t staging_bytes(const fsg_ctx *c, int64_t n) {
+    const size_t pb = (size_t)n * 16, zb = z_bytes(c, n), ub = (size_t)n * 4, rb = (size_t)n;
+    return align256(pb) + align256(zb) + 2 * align256(ub) + align256(rb);
+}
+
+// one tile through device staging at b (staging_bytes(c, n) bytes) with the
+// cursor and counters at work; the caller holds c->mu
+int tile_staged(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, double *Z, int32_t *U,
+                int8_t *stop_reason, int32_t *stop_iter, const volatile uint8_t *interrupted,
+                fsg_stats *stats, char *b, unsigned long long *work) {
+    const size_t pb = (size_t)n * 16, zb = z_bytes(c, n), ub = (size_t)n * 4, rb = (size_t)n;
+    const size_t off_z = align256(pb), off_u = off_z + align256(zb), off_i = off_u + align256(ub);
+    const size_t off_r = off_i + align256(ub);
+    if (n > 0) FSG_HIP(hipMemcpyAsync(b, c_pix, pb, hipMemcpyHostToDevice, c->stream));
+    int rc = launch(c, reinterpret_cast<double *>(b), n, reinterpret_cast<double *>(b + off_z),
+                    reinterpret_cast<int32_t *>(b + off_u), reinterpret_cast<int8_t *>(b + off_r),
+                    reinterpret_cast<int32_t *>(b + off_i), work, c->stream, true, -1, 0, grid_w);
+    if (rc) return rc;
+    if (n > 0) {
+        FSG_HIP(hipMemcpyAsync(Z, b + off_z, zb, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(U, b + off_u, ub, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(stop_iter, b + off_i, ub, hipMemcpyDeviceToHost, c->stream));
+        FSG_HIP(hipMemcpyAsync(stop_reason, b + off_r, rb, hipMemcpyDeviceToHost, c->stream));
+    }
+    FSG_HIP(hipStreamSynchronize(c->stream));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        unsigned long long w[WORK_WORDS];
+        FSG_HIP(hipMemcpy(w, work, sizeof(w), hipMemcpyDeviceToHost));
+        stats->plain_steps = w[1]; stats->bla_steps = w[2]; stats->bla_iters = w[3]; stats->rebases = w[4];
+        stats->bla_lookups = w[5];
+        float ms = 0.f;
+        if (n > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) stats->kernel_ms = ms;
+    }
+    if (interrupted && *interrupted) return FSG_INTERRUPTED;
+    return 0;
+}
+
+// ---- the staging of fsg_tile_grid_noalloc -------------------------------
+// A file-scope device array: every device the library runs on holds its own
+// copy for the life of the process, and no call allocates.  Its address on a
+// device comes from hipGetSymbolAddress, taken once per device with that
+// device current; host code uses that address, never the array's name.
+constexpr size_t NOALLOC_WORK_BYTES = 256;       // cursor and counters, then the tile staging
+static __device__ __align__(256) unsigned char scratch[FSG_NOALLOC_SCRATCH_BYTES];
+
+struct ScratchDev {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    char *ptr = nullptr;
+    std::mutex mu;                               // one tile at a time per device
+};
+constexpr int MAX_DEVICES = 64;
+ScratchDev g_scratch[MAX_DEVICES];
+
+int scratch_of(int dev, char **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(FSG_E_INVALID, "device id out of range");
+    ScratchDev &s = g_scratch[dev];
+    std::call_once(s.once, [&] {
+        int cur = -1;
+        s.err = hipGetDevice(&cur);
+        if (s.err != hipSuccess) return;
+        if (cur != dev && (s.err = hipSetDevice(dev)) != hipSuccess) return;
+        void *p = nullptr;
+        s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (s.err == hipSuccess) s.ptr = static_cast<char *>(p);
+        if (cur != dev) (void)hipSetDevice(cur);
+    });
+    if (s.err != hipSuccess || !s.ptr)
+        return fail(FSG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(s.err));
+    *out = s.ptr;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fsg_abi_version(void) { return FSG_ABI_VERSION; }
@@ -614,36 +694,38 @@ int fsg_tile_grid(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, do
     if (grid_w < 0) return fail(FSG_E_INVALID, "negative grid width");
     if (interrupted && *interrupted) return FSG_INTERRUPTED;
     std::lock_guard<std::mutex> lk(c->mu);
-    const size_t pb = (size_t)n * 16, zb = z_bytes(c, n), ub = (size_t)n * 4, rb = (size_t)n;
-    const size_t off_z = align256(pb), off_u = off_z + align256(zb), off_i = off_u + align256(ub);
-    const size_t off_r = off_i + align256(ub), total = off_r + align256(rb);
+    const size_t total = staging_bytes(c, n);
     if (total > c->buf_bytes) {
         if (c->buf) { (void)hipFree(c->buf); c->buf = nullptr; c->buf_bytes = 0; }
         if (hipMalloc(&c->buf, total) != hipSuccess) { c->buf = nullptr; return fail(FSG_E_NOMEM, "hipMalloc tile buffers"); }
         c->buf_bytes = total;
     }
-    char *b = c->buf;
-    if (n > 0) FSG_HIP(hipMemcpyAsync(b, c_pix, pb, hipMemcpyHostToDevice, c->stream));
-    int rc = launch(c, reinterpret_cast<double *>(b), n, reinterpret_cast<double *>(b + off_z),
-                    reinterpret_cast<int32_t *>(b + off_u), reinterpret_cast<int8_t *>(b + off_r),
-                    reinterpret_cast<int32_t *>(b + off_i), c->work, c->stream, true, -1, 0, grid_w);
-    if (rc) return rc;
-    if (n > 0) {
-        FSG_HIP(hipMemcpyAsync(Z, b + off_z, zb, hipMemcpyDeviceToHost, c->stream));
-        FSG_HIP(hipMemcpyAsync(U, b + off_u, ub, hipMemcpyDeviceToHost, c->stream));
-        FSG_HIP(hipMemcpyAsync(stop_iter, b + off_i, ub, hipMemcpyDeviceToHost, c->stream));
-        FSG_HIP(hipMemcpyAsync(stop_reason, b + off_r, rb, hipMemcpyDeviceToHost, c->stream));
-    }
-    FSG_HIP(hipStreamSynchronize(c->stream));
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        int r2 = fsg_work_stats(c->work, stats);
-        if (r2) return r2;
-        float ms = 0.f;
-        if (n > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) stats->kernel_ms = ms;
-    }
+    return tile_staged(c, c_pix, n, grid_w, Z, U, stop_reason, stop_iter, interrupted, stats, c->buf, c->work);
+}
+
+int fsg_tile_grid_noalloc(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, double *Z,
+                          int32_t *U, int8_t *stop_reason, int32_t *stop_iter,
+                          const volatile uint8_t *interrupted, fsg_stats *stats) {
+    if (!c) return fail(FSG_E_INVALID, "null context");
+    if (n < 0 || (n > 0 && (!c_pix || !Z || !U || !stop_reason || !stop_iter)))
+        return fail(FSG_E_INVALID, "bad tile buffers");
+    if (grid_w < 0) return fail(FSG_E_INVALID, "negative grid width");
+    const size_t need = NOALLOC_WORK_BYTES + staging_bytes(c, n);
+    if (need > FSG_NOALLOC_SCRATCH_BYTES)
+        return fail(FSG_E_RANGE, "fsg_tile_grid_noalloc: a tile of " + std::to_string(n) + " pixels needs " +
+                                 std::to_string(need) + " bytes of device staging, above the " +
+                                 std::to_string((size_t)FSG_NOALLOC_SCRATCH_BYTES) +
+                                 " bytes this entry point holds per device; use fsg_tile_grid or smaller tiles");
     if (interrupted && *interrupted) return FSG_INTERRUPTED;
-    return 0;
+    char *base = nullptr;
+    int rc = scratch_of(c->device, &base);
+    if (rc) return rc;
+    // one staging area per device: calls on contexts of the same device
+    // take turns (the call is synchronous)
+    std::lock_guard<std::mutex> lk_dev(g_scratch[c->device].mu);
+    std::lock_guard<std::mutex> lk(c->mu);
+    return tile_staged(c, c_pix, n, grid_w, Z, U, stop_reason, stop_iter, interrupted, stats,
+                       base + NOALLOC_WORK_BYTES, reinterpret_cast<unsigned long long *>(base));
 }
 
 int fsg_tile_device(fsg_ctx *c, const double *d_c_pix, int64_t n, double *d_Z, int32_t *d_U,

@@ -186,6 +186,24 @@ int fsg_tile_grid_device(fsg_ctx *ctx, const double *d_c_pix, int64_t n, int64_t
                          double *d_Z, int32_t *d_U, int8_t *d_stop_reason, int32_t *d_stop_iter,
                          void *d_work, void *stream);
 
+/*
+ * fsg_tile_grid without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture elsewhere in the process, serving loops that forbid
+ * allocation after start-up).  The device staging of the inputs and outputs
+ * and the pixel cursor and counters live in a static per-device array of
+ * FSG_NOALLOC_SCRATCH_BYTES (32 MiB), held by every device the library runs
+ * on until the process ends; its address is taken once per device.  A tile
+ * needing more staging than that (about 340,000 pixels with three complex
+ * fields) is rejected with FSG_E_RANGE.  Calls on one device share the array
+ * and take turns (the call is synchronous); a call leaves its data there and
+ * the next call overwrites it.  Results equal fsg_tile_grid's.
+ */
+#define FSG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int fsg_tile_grid_noalloc(fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w, double *Z,
+                          int32_t *U, int8_t *stop_reason, int32_t *stop_iter,
+                          const volatile uint8_t *interrupted, fsg_stats *stats);
+
 /* Size of the `work` buffer of fsg_tile_device. No reference counterpart. */
 size_t fsg_work_bytes(void);
 
