@@ -44,6 +44,19 @@ for _name, _e in _EX.items():
                      dict(max_iter=min(_e["max_iter"], 20000)))
     FRAMES[_name + "_flavor"] = _e["flavor"]
 
+# the reference's edge cases (tests/golden/edge_frames.json, written by
+# make_edge_frames.py): options of the test's calc helper
+# (test_perturbation.py:717-760): calc_fast frames without dz/dc, the others
+# with calc_orbit and backshift 2
+EDGE = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                   "golden", "edge_frames.json")))
+for _name, _e in EDGE.items():
+    _full = not _e["calc_fast"]
+    FRAMES[_name] = ("M2", dict(x=_e["x"], y=_e["y"], dx=_e["dx"], precision=_e["precision"],
+                                xy_ratio=_e["xy_ratio"]),
+                     dict(max_iter=_e["max_iter"], interior_detect=_e["interior_detect"],
+                          calc_dzndc=_full, calc_orbit=_full, backshift=2 if _full else 0))
+
 # z^d + c: seeded boundary-descent centres (fractalshades_amd.configs)
 MN_TEST = {d: configs.Workload(f"T_MN{d}", "MN", "1e-14", 64, 64, 5000, 40, degree=d,
                                period_range=(10, 3000), seed=d) for d in (2, 3, 4, 5, 6, 7, 8)}
@@ -54,7 +67,7 @@ def mn_frame(d, nx=48, xr=False, **calc_over):
     if d not in _mn_centers:
         _mn_centers[d] = configs.seeded_center(MN_TEST[d])
     x, y, _, _ = _mn_centers[d]
-    f = fsa.Perturbation_mandelbrot_N("/tmp/fsa_test", d)
+    f = fsa.Perturbation_mandelbrot_N(None, d)
     f.zoom(precision=40, x=x, y=y, dx="1e-14", nx=nx, xy_ratio=1.0, theta_deg=0.)
     kw = dict(calc_name="t", subset=None, max_iter=5000, M_divergence=1.e3,
               epsilon_stationnary=1.e-3, BLA_eps=1.e-6, interior_detect=True)
@@ -70,12 +83,12 @@ def make(name, nx=48, model_kw=None, no_newton=False, **calc_over):
     if model == "BS" and model_kw is None:
         model_kw = {"flavor": FRAMES[name + "_flavor"]}
     if model == "M2":
-        f = fsa.Perturbation_mandelbrot("/tmp/fsa_test")
+        f = fsa.Perturbation_mandelbrot(None)
     elif model == "MN":
-        f = fsa.Perturbation_mandelbrot_N("/tmp/fsa_test", **(model_kw or {}))
+        f = fsa.Perturbation_mandelbrot_N(None, **(model_kw or {}))
     else:
-        f = fsa.Perturbation_burning_ship("/tmp/fsa_test", **(model_kw or {}))
-    f.zoom(nx=nx, xy_ratio=1.0, theta_deg=0., **zoom_kw)
+        f = fsa.Perturbation_burning_ship(None, **(model_kw or {}))
+    f.zoom(**dict(dict(nx=nx, xy_ratio=1.0, theta_deg=0.), **zoom_kw))
     f.no_newton = no_newton
     kw = dict(calc_name="t", subset=None, M_divergence=1.e3, epsilon_stationnary=1.e-3,
               BLA_eps=1.e-6)
@@ -93,7 +106,8 @@ def frame_inputs(f, calc_name="t"):
     return args, np.ascontiguousarray(c_pix)
 
 
-def expmap_frame(name="M2_E20", nx=60, hmin=0., hmax=10., dx=None, orientation="horizontal", **calc_over):
+def expmap_frame(name="M2_E20", nx=60, hmin=0., hmax=10., dx=None, orientation="horizontal",
+                 precision=None, no_newton=False, x=None, y=None, **calc_over):
     """A test frame's location seen through the exponential map
     exp(h + i t), h in [hmin, hmax] (fractalshades_amd.projection.Expmap)."""
     from fractalshades_amd.projection import Expmap
@@ -101,11 +115,16 @@ def expmap_frame(name="M2_E20", nx=60, hmin=0., hmax=10., dx=None, orientation="
     zoom_kw = dict(zoom_kw, projection=Expmap(hmin, hmax, orientation=orientation))
     if dx is not None:
         zoom_kw["dx"] = dx
+    if precision is not None:
+        zoom_kw["precision"] = precision
+    if x is not None:
+        zoom_kw["x"], zoom_kw["y"] = x, y
     if model == "M2":
-        f = fsa.Perturbation_mandelbrot("/tmp/fsa_test")
+        f = fsa.Perturbation_mandelbrot(None)
     else:
-        f = fsa.Perturbation_burning_ship("/tmp/fsa_test", flavor=FRAMES[name + "_flavor"])
+        f = fsa.Perturbation_burning_ship(None, flavor=FRAMES[name + "_flavor"])
     f.zoom(nx=nx, xy_ratio=1.0, theta_deg=0., **zoom_kw)
+    f.no_newton = no_newton
     kw = dict(calc_name="t", subset=None, M_divergence=1.e3, epsilon_stationnary=1.e-3,
               BLA_eps=1.e-6)
     if model == "BS":

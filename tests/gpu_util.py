@@ -25,4 +25,13 @@ def assert_same(got, ref):
     np.testing.assert_array_equal(srg, srr)
     np.testing.assert_array_equal(Ug, Ur)
     # bit-exact (NaN-aware) on every output field
-    np.testing.assert_array_equal(Zg.view(np.uint64), Zr.view(np.uint64))
+    bad = Zg.view(np.uint64) != Zr.view(np.uint64)
+    if np.any(bad):
+        rows, cols = np.nonzero(bad.reshape(Zg.view(np.uint64).shape[0], -1))
+        k = cols[:6]
+        flat_g = Zg.view(np.float64).reshape(Zg.shape[0], -1)
+        flat_r = Zr.view(np.float64).reshape(Zr.shape[0], -1)
+        detail = [(int(r), int(c), float(flat_g[r, c]), float(flat_r[r, c]))
+                  for r, c in zip(rows[:6], cols[:6])]
+        raise AssertionError(f"{int(bad.sum())} of {bad.size} values differ; rows {sorted(set(rows.tolist()))}; "
+                             f"(row, float index, got, want): {detail}; stop_iter {sig.reshape(-1)[k // 2 if np.iscomplexobj(Zg) else k].tolist()}")

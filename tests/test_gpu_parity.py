@@ -284,3 +284,83 @@ def test_expmap_steps_bitexact(orientation, monkeypatch):
                         for a in (res["Z"], res["U"], res["stop_reason"], res["stop_iter"]))
             assert_same(got, ref)
     assert len(kcs) >= 3           # each band has its own BLA bound
+
+
+def test_deep_expmap_bitexact():
+    """A deep exponential map (h in [0, 758] at dx = 1e-330, exp(hmax) above
+    1e300: the reference's Xrange h) through the product path: calc_raw runs
+    it by bands of h, each band's offsets relative to exp(h_band) and its
+    pixel scale dx exp(h_band); every band's pixels equal the oracle run with
+    that band's descriptor, offsets and modifier."""
+    f = frames.expmap_frame("M2_E20", nx=1200, hmin=0., hmax=758., dx="1e-330", precision=345,
+                            no_newton=True, max_iter=3000, x="-0.75", y="0.2")
+    assert f.projection.deep
+    print("case deep expmap", flush=True)
+    res = f.calc_raw("t")
+    nx, ny = f.nx, f.ny
+    proj = f.projection
+    n_bands = 0
+    for fields, arrays, chunks in f.expmap_bands("t"):
+        n_bands += 1
+        for ix, ixx, iy, iyy in chunks:
+            c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((ix, ixx, iy, iyy))))
+            ref = orc.OracleCtx(fields, arrays).cycles(np.ascontiguousarray(proj.f(c_pix)))
+            mod = proj.dzndc_modifier(c_pix)
+            for row in (1,):
+                ref[0][row] *= mod
+            sel = (slice(None), slice(iy, iyy), slice(ix, ixx))
+            got = tuple(np.ascontiguousarray(a.reshape(a.shape[0], ny, nx)[sel]).reshape(a.shape[0], -1)
+                        for a in (res["Z"], res["U"], res["stop_reason"], res["stop_iter"]))
+            assert_same(got, ref)
+    assert n_bands >= 5
+    assert len(np.unique(res["stop_iter"])) >= 2
+
+
+@pytest.mark.parametrize("model", ["M2", "Mn3"])
+@pytest.mark.parametrize("interior", [0, 1])
+def test_critical_orbit_bitexact(model, interior):
+    """Shallow frames (width above 1e-5) run around the critical orbit with
+    BLA off, as the reference does (perturbation.py:669-672): the orbit is
+    constant 0, so every step rebases; with interior detection the dz/dz
+    path's wrapped value sits at [L] with L = order + 1."""
+    import fractalshades_amd as fsa
+    f = fsa.Perturbation_mandelbrot(None) if model == "M2" else fsa.Perturbation_mandelbrot_N(None, 3)
+    f.zoom(precision=20, x="-0.6", y="0.1", dx="2.5", nx=48,
+           xy_ratio=1.0, theta_deg=0.)
+    f.calc_std_div(calc_name="t", subset=None, max_iter=3000, M_divergence=1.e3,
+                   epsilon_stationnary=1.e-3, BLA_eps=1.e-6, interior_detect=bool(interior))
+    args, c_pix = frames.frame_inputs(f)
+    assert args.fields["ref_order"] == 100 and args.fields["bla_on"] == 0
+    print(f"case critical {model} interior={interior}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, _ = run_gpu(args, c_pix)
+    assert_same(got, ref)
+
+
+# the reference's edge cases (test_perturbation.py:203-297, 376-420, 503-576,
+# 651-716) at reduced size: deep zoom with calc_orbit, Dinkydau's flake (the
+# dynamic-glitch case), an Xrange frame at 5.07e-433 with interior
+# detection, and 3x3 supersampling (with and without jitter)
+EDGE_RUNS = {"M2_E213": dict(nx=48), "M2_flake": dict(nx=64),
+             "M2_ultradeep": dict(nx=40, max_iter=2_000_000), "M2_supersampling": dict(nx=24)}
+
+
+@pytest.mark.parametrize("jitter", [0, 1])
+@pytest.mark.parametrize("name", list(EDGE_RUNS))
+def test_edge_frames_bitexact(name, jitter):
+    if jitter and name != "M2_supersampling":
+        pytest.skip("jitter is run on the supersampled frame")
+    f = frames.make(name, **EDGE_RUNS[name])
+    args = f._calc_data["t"]["cycle_indep_args"]
+    ssg = frames.EDGE[name]["supersampling"]
+    c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny), jitter, ssg)))
+    print(f"case edge {name} jitter={jitter} npts={c_pix.size}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+    got_grid, _ = run_gpu(args, c_pix, grid_w=f.nx * (ssg or 1))
+    assert_same(got_grid, ref)
+    # the product path (calc_raw) gives the same pixels
+    res = f.calc_raw("t", jitter=jitter, supersampling=ssg)
+    np.testing.assert_array_equal(res["stop_iter"], ref[3])
+    np.testing.assert_array_equal(res["Z"].view(np.uint64), ref[0].view(np.uint64))

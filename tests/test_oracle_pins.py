@@ -186,3 +186,58 @@ def test_deep_frame_vs_brute_force(name):
         rebases += int(off[1][0, i] != off[3][0, i])
     # the sample crosses at least one rebase of the reference orbit
     assert rebases >= 1
+
+
+# ---- the reference's edge-case frames -----------------------------------------
+EDGE = {"M2_E213": (350000, 3), "M2_flake": (50000, 4), "M2_ultradeep": (2_000_000, 1)}
+_SLOW = pytest.mark.skipif(not __import__("os").environ.get("FSG_SLOW"),
+                           reason="brute force over 231,000 iterations at 1950 bits (~40 s); FSG_SLOW=1 runs it")
+
+
+@pytest.mark.parametrize("name", ["M2_E213", "M2_flake",
+                                  pytest.param("M2_ultradeep", marks=[pytest.mark.slow, _SLOW])])
+def test_edge_frame_vs_brute_force(name):
+    """Deep zoom at 3.2e-213 (~134,000 iterations per pixel), the flake
+    (dynamic glitches, ~31,000) and the Xrange interior frame at 5.07e-433
+    (escapes from 231,000 iterations): on the earliest-escaping pixels of a
+    40-pixel-wide version, the escape count with BLA off is the brute-force
+    one, with BLA on within one iteration; z_n within 1e-6."""
+    max_iter, k = EDGE[name]
+    f = frames.make(name, nx=40, max_iter=max_iter)
+    args, c_pix = frames.frame_inputs(f)
+    off = orc.OracleCtx(dict(args.fields, bla_on=0), args.arrays).cycles(c_pix)
+    on = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    esc = np.flatnonzero(off[2][0] == 1)
+    for i in esc[np.argsort(off[3][0, esc], kind="stable")][:k]:
+        n, z, _ = brute.holo_escape(f, c_pix[i], 2, max_iter, 1e3, prec=f.prec + 128)
+        assert n == off[3][0, i], (i, n, off[3][0, i])
+        assert abs(int(n) - int(on[3][0, i])) <= 1
+        assert rel(off[0][0, i], z) <= 1e-6, (i, off[0][0, i], z)
+    if name == "M2_ultradeep":
+        assert np.count_nonzero(on[2] == 2) > 0 and args.fields["xr_mode"] == 1
+
+
+@pytest.mark.parametrize("jitter", [0, 1])
+def test_supersampling_pixel_offsets(jitter):
+    """chunk_pixel_pos with 3x3 supersampling (core.py:1767-1830, the
+    reference's test_supersampling, test_perturbation.py:376-420): each
+    pixel's 3x3 sub-pixels are centred on it, a third of a pixel apart, rows
+    flipped; jitter moves each by at most half its rescaled amplitude,
+    reproducibly (seeded)."""
+    f = frames.make("M2_supersampling", nx=12)
+    nx, ny, s = f.nx, f.ny, 3
+    plain = f.chunk_pixel_pos((0, nx, 0, ny))
+    ss = f.chunk_pixel_pos((0, nx, 0, ny), 0, s)
+    assert ss.shape == (ny * s, nx * s)
+    blocks = ss.reshape(ny, s, nx, s).mean(axis=(1, 3))
+    np.testing.assert_allclose(blocks, plain, atol=1e-15)
+    step_x = 1. / (nx - 1) / s
+    np.testing.assert_allclose(np.diff(ss.real, axis=1), step_x, rtol=1e-9)
+    np.testing.assert_allclose(np.diff(ss.imag, axis=0), -1. / (ny - 1) / s / f.xy_ratio, rtol=1e-9)
+    if jitter:
+        jt = f.chunk_pixel_pos((0, nx, 0, ny), 1, s)
+        d = jt - ss
+        amp = 0.5 * 0.7071067811865476 / (nx - 1) / s
+        assert np.all(np.abs(d.real) <= amp) and np.all(np.abs(d.imag) <= amp)
+        assert np.std(d.real) > amp / 4
+        np.testing.assert_array_equal(jt, f.chunk_pixel_pos((0, nx, 0, ny), 1, s))
