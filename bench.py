@@ -66,6 +66,18 @@ def flops_per_unit(fields):
     return plain, bla, 4, reb
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name")."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
     """Oracle on a strided sample of the frame: grow the sample until it
     takes about `budget_s` seconds."""
@@ -86,6 +98,7 @@ def cpu_baseline(fields, arrays, c_pix, budget_s=15.0):
         "value": float(si.astype(np.int64).sum() / dt),
         "unit": "pixel-iters/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"every {stride}th pixel of the frame ({sample.size} px), {dt:.1f} s, "
                   "C restatement of the reference numba loop, OpenMP dynamic schedule",
@@ -106,7 +119,7 @@ def glitch_report(res, fields):
     return {"unfinished": int(np.count_nonzero(sr < 0)), "non_finite": int(np.count_nonzero(nonfinite))}
 
 
-def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
+def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False, exterior=False):
     """The request stream: seeded requests of mixed size / width / max_iter
     on eight centres (fractalshades_amd.serving), first all at once (the
     makespan gives the capacity), then with Poisson arrivals at `rate_frac`
@@ -115,6 +128,8 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
     from fractalshades_amd.postproc import post_desc
     t0 = time.perf_counter()
     centres = serving.serve_centres()
+    if exterior:
+        centres = [serving.boundary_centre()]
     specs = serving.serve_requests(n_requests, len(centres), seed=seed)
     bases = {}
     reqs = []
@@ -123,7 +138,7 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
         f, cia, c_pix = serving.build_request(sp, centres, precision)
         if layers:
             posts.append(post_desc(f, "serve"))
-        key = (sp.centre, cia.fields["xr_mode"])
+        key = cia.resident_key
         if key not in bases:
             bases[key] = cia.ctx            # resident orbit + BLA coefficients of this centre
         else:
@@ -157,6 +172,8 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False):
     span = (max(r["t_done_ms"] for r in steady) - min(r["t_submit_ms"] for r in steady)) * 1e-3
     out = {
         "requests": n_requests, "precision": precision, "host_prep_s": round(prep_s, 1),
+        "frames": "one boundary centre (the seeded 1e-2000 location), exterior-dominated" if exterior
+                  else "eight nuclei, offset within 0.3 widths",
         "returns": "nu/dem float32 layers + stop reason/iteration (13 B/px)" if layers
                    else "raw Z/U/stop reason/iteration (57 B/px)",
         "pixels": int(sum(r["pixels"] for r in burst)), "pixel_iters": int(pix_iters),
@@ -187,7 +204,11 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
     centre = serving.movie_centre(max_iter)
     orbit_s = time.perf_counter() - t0
     specs = serving.movie_requests(n_frames, nx, ny, max_iter)
-    out = {"frames": n_frames, "shape": f"{nx}x{ny}", "max_iter": max_iter,
+    out = {"workload": f"zoom sequence: {n_frames} of the 2000 frames of the full sequence (same width "
+                       f"schedule, every {2000 // max(n_frames, 1)}th frame), {nx}x{ny} px, max_iter "
+                       f"{max_iter}, one resident unwrapped orbit of {max_iter + 1} points; fewer frames "
+                       "because each frame's derivative paths over the whole orbit are prepared on the host",
+           "frames": n_frames, "shape": f"{nx}x{ny}", "max_iter": max_iter,
            "widths": f"{specs[0].width} .. {specs[-1].width}", "orbit_points": max_iter + 1,
            "orbit_host_s": round(orbit_s, 1)}
     results = {}
@@ -197,7 +218,7 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
         for sp in specs:
             f, cia, c_pix = serving.build_request(sp, [centre], precision)
             posts.append(post_desc(f, "serve"))
-            key = cia.fields["xr_mode"]
+            key = cia.resident_key
             if key not in bases:
                 bases[key] = cia.ctx
             else:
@@ -311,12 +332,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="C2")
     ap.add_argument("--deep", default="C2_2000", help="second frame (1e-2000); '' to skip")
-    ap.add_argument("--extra", default="C3_d3,C3_d4,C3_d6,C3_bs",
+    ap.add_argument("--extra", default="C2_int,C3_d3,C3_d4,C3_d6,C3_bs",
                     help="further workloads reported under config.frames (fewer steps); '' to skip")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
     ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
-    ap.add_argument("--movie", type=int, default=64, help="frames of the zoom-sequence leg (0: skip)")
+    ap.add_argument("--movie", type=int, default=24,
+                    help="frames of the zoom-sequence leg at 1920x1080, max_iter 5e6, 5e6-point orbit (0: skip)")
     ap.add_argument("--order", choices=("grid", "linear"), default="grid",
                     help="pixel order of the kernel: 8x8 blocks of the frame, or index order")
     ap.add_argument("--calib-copy", action="store_true",
@@ -412,7 +434,8 @@ def main():
     }
     if args.serve > 0:
         serve = {"fp64_layers": serving_leg(args.serve, "fp64", layers=True),
-                 "fp64_raw": serving_leg(args.serve, "fp64")}
+                 "fp64_raw": serving_leg(args.serve, "fp64"),
+                 "fp64_layers_exterior": serving_leg(args.serve, "fp64", layers=True, exterior=True)}
         if not args.no_fp32x:
             serve["fp32x_layers"] = serving_leg(args.serve, "fp32x", layers=True)
         out["config"]["serving"] = serve
@@ -420,7 +443,7 @@ def main():
         if lat512:
             out["tile_latency_512sq_ms"] = {"p50": lat512["p50_ms"], "p99": lat512["p99_ms"]}
     if args.movie > 0:
-        out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=960, ny=540)
+        out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=1920, ny=1080, max_iter=5_000_000)
     if not args.no_cpu:
         base, stride, ref = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
         # the same sample checked against the GPU's results of the timed frame

@@ -63,6 +63,13 @@ WORKLOADS = {
     "C2_2000": Workload("C2_2000", "M2", "1e-2000", 2048, 2048, 200000, 2040,
                         interior_detect=True, period_range=(50, 20000),
                         extra={"no_newton": True}),
+    # interior-dominated variant of C2: the frame centred on C2's seeded
+    # nucleus (period 1079) at 1.3x its component size (nucleus_size_estimate:
+    # 3.9e-601), so that the minibrot fills the frame's centre; the precision
+    # lets the reference's Newton converge, the orbit is the period-1079
+    # cycle and the interior pixels stop on the dz/dz test
+    "C2_int": Workload("C2_int", "M2", "5e-601", 2048, 2048, 200000, 640, interior_detect=True,
+                       period_range=(50, 2000), extra={"interior_of": "C2"}),
     "C3_d3": Workload("C3_d3", "MN", "1e-800", 1920, 1080, 1000000, 850, degree=3,
                       period_range=(50, 5000)),
     "C3_d4": Workload("C3_d4", "MN", "1e-800", 1920, 1080, 1000000, 850, degree=4,
@@ -74,7 +81,7 @@ WORKLOADS = {
 }
 
 
-def new_model(w, directory="/tmp/fractalshades_amd"):
+def new_model(w, directory=None):
     if w.model == "M2":
         return models.Perturbation_mandelbrot(directory)
     if w.model == "MN":
@@ -313,7 +320,21 @@ def center_for(name, compute=True):
         return ent["x"], ent["y"], nucleus, ent["order"]
     if not compute:
         raise KeyError(name)
+    if "interior_of" in w.extra:
+        return interior_center(w)
     return seeded_center(w)
+
+
+def interior_center(w):
+    """(x, y, nucleus, order) of an interior-dominated workload: the nucleus
+    of the base workload's seeded centre, refined by Newton to the
+    workload's precision, is the frame centre."""
+    from . import fixed_orbit
+    _, _, nucleus, order = center_for(w.extra["interior_of"], compute=False)
+    prec = mpmath.libmp.dps_to_prec(w.digits + 60)
+    with mpmath.workprec(prec):
+        _, nuc = fixed_orbit.find_nucleus_m2(mpmath.mpc(nucleus), order, mpmath.mpf(w.width), prec)
+        return mpmath.nstr(nuc.real, w.digits + 5), mpmath.nstr(nuc.imag, w.digits + 5), nuc, order
 
 
 def write_centers(names):
@@ -324,7 +345,7 @@ def write_centers(names):
         cache = {}
     for name in names:
         w = WORKLOADS[name]
-        x, y, nucleus, order = seeded_center(w)
+        x, y, nucleus, order = interior_center(w) if "interior_of" in w.extra else seeded_center(w)
         cache[name] = {"key": _key(w), "x": x, "y": y, "order": order,
                        "nucleus_x": mpmath.nstr(nucleus.real, w.digits + 5),
                        "nucleus_y": mpmath.nstr(nucleus.imag, w.digits + 5)}

@@ -135,6 +135,8 @@ struct Req {
     hipEvent_t h2d = nullptr, d2h = nullptr;
     int state = PENDING;
     int status = 0;
+    int waiters = 0;          // threads inside fsg_server_wait on this request
+    bool released = false;    // its ticket has been returned by a wait
     std::string msg;   // error message of a failed request (the scheduler's thread-local one)
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
 };
@@ -479,9 +481,19 @@ int fsg_server_wait(fsg_server *s, uint64_t ticket, int32_t timeout_ms, fsg_requ
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return set_error(FSG_E_INVALID, "unknown ticket");
     Req *r = it->second;
+    // several threads may wait on one ticket: the request lives until the
+    // last of them leaves; the first to see it done releases the ticket
+    r->waiters++;
     auto done = [&] { return r->state == DONE; };
+    bool ok = true;
     if (timeout_ms < 0) s->cv_done.wait(lk, done);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) return FSG_PENDING;
+    else ok = s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done);
+    r->waiters--;
+    if (!ok) return FSG_PENDING;
+    if (!r->released) {
+        r->released = true;
+        s->reqs.erase(ticket);
+    }
     if (info) {
         info->status = r->status;
         info->chunks = r->chunks;
@@ -493,8 +505,7 @@ int fsg_server_wait(fsg_server *s, uint64_t ticket, int32_t timeout_ms, fsg_requ
     }
     const int st = r->status;
     if (st != 0) set_error(st, r->msg);
-    s->reqs.erase(it);
-    delete r;
+    if (r->waiters == 0) delete r;
     return st;
 }
 

@@ -7,7 +7,9 @@ compute_FP_orbit 808-854, ref_point_matching 211-253), path data
 preparation ``get_cycle_indep_args`` (431-562).  The per-tile loop
 (``numba_cycle_call``, 414-428) runs on the GPU through libfsgpu.so.
 """
+import json
 import logging
+import os
 
 import mpmath
 import numpy as np
@@ -26,6 +28,12 @@ REF_ORDER_NONE = 1 << 62   # perturbation.py:455-456
 PRECISIONS = {"fp64": FSG_PREC_FP64, "fp32x": FSG_PREC_FP32X}
 
 
+def _without_dir(kw):
+    """Model parameters other than the directory (the reference compares
+    those only, perturbation.py:219-220)."""
+    return {k: v for k, v in kw.items() if k != "directory"}
+
+
 class CycleIndepArgs:
     """What the per-tile call needs, computed once per calculation: the
     engine descriptor (fields + arrays) and, created on first use, the
@@ -38,6 +46,7 @@ class CycleIndepArgs:
         self.last_stats = None
         self.share = share      # GpuContext whose orbit / BLA coefficients to reuse
         self.projection = projection   # non-identity screen projection, applied on the host
+        self.resident_key = None       # identity of the orbit, for sharing its device copy
         self._ctx = None
 
     @property
@@ -53,13 +62,17 @@ class CycleIndepArgs:
 
     def with_fields(self, **over):
         """A copy with some descriptor fields replaced (its own context)."""
-        return CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted, self.share,
-                              self.projection)
+        c = CycleIndepArgs(dict(self.fields, **over), self.arrays, self.interrupted, self.share,
+                           self.projection)
+        c.resident_key = self.resident_key
+        return c
 
     def shared_with(self, base_ctx):
         """A copy whose context reuses base_ctx's device orbit and BLA
         coefficients (same orbit required)."""
-        return CycleIndepArgs(self.fields, self.arrays, self.interrupted, base_ctx, self.projection)
+        c = CycleIndepArgs(self.fields, self.arrays, self.interrupted, base_ctx, self.projection)
+        c.resident_key = self.resident_key
+        return c
 
     def dzndc_rows(self):
         """Rows of the raw Z array that hold dz/dc (the burning ship's four
@@ -126,16 +139,86 @@ class PerturbationFractal(Fractal):
     # ---- reference orbit ------------------------------------------------------
     @property
     def FP_params(self):
+        """The reference point's parameters, reloaded from the directory's
+        stored orbit when none was computed in this session
+        (perturbation.py:292-311)."""
+        if self._FP_params is None:
+            self.reload_ref_point()
         return self._FP_params
 
     @property
     def Zn_path(self):
+        if self._Zn_path is None:
+            self.reload_ref_point()
         return self._Zn_path
+
+    def ref_point_file(self):
+        """Base path of the stored reference orbit: <directory>/data/ref_pt
+        (.json parameters and Xrange register, .npy orbit).  The reference
+        pickles the same content to ref_pt.dat (perturbation.py:256-290); here
+        nothing executable is stored or loaded."""
+        if self.directory is None:
+            return None
+        return os.path.join(self.directory, "data", "ref_pt")
+
+    def save_ref_point(self, FP_params, Zn_path):
+        """Store the orbit and its parameters for later calculations of the
+        same location (save_ref_point, perturbation.py:256-274)."""
+        self._FP_params = FP_params
+        self._Zn_path = Zn_path
+        base = self.ref_point_file()
+        if base is None:
+            return
+        os.makedirs(os.path.dirname(base), exist_ok=True)
+        ref = FP_params["ref_point"]
+        with mpmath.workprec(FP_params["prec"]):
+            ref = mpmath.mpc(ref)
+            # exact: signed mantissa and binary exponent of each part
+            pt = []
+            for v in (ref.real, ref.imag):
+                sign, man, exp, _ = v._mpf_
+                pt.append((-int(man) if sign else int(man), int(exp)))
+        if self.holomorphic:
+            xr = [[int(i), float(v[0]), float(v[1]), int(v[2])] for i, v in sorted(FP_params["xr"].items())]
+        else:
+            xr = [[int(i), float(v[0][0]), int(v[0][1]), float(v[1][0]), int(v[1][1])]
+                  for i, v in sorted(FP_params["xr"].items())]
+        meta = {k: v for k, v in FP_params.items() if k not in ("ref_point", "xr")}
+        meta.update(ref_point=[[int(m), int(e)] for m, e in pt], xr=xr,
+                    holomorphic=bool(self.holomorphic))
+        np.save(base + ".npy.tmp.npy", np.ascontiguousarray(Zn_path, dtype=np.complex128),
+                allow_pickle=False)
+        os.replace(base + ".npy.tmp.npy", base + ".npy")
+        with open(base + ".json.tmp", "w") as fh:
+            json.dump(meta, fh)
+        os.replace(base + ".json.tmp", base + ".json")
+
+    def reload_ref_point(self):
+        """Reload the stored orbit, if any (reload_ref_point,
+        perturbation.py:277-290).  Returns True when one was found."""
+        base = self.ref_point_file()
+        if base is None or not (os.path.exists(base + ".json") and os.path.exists(base + ".npy")):
+            return False
+        with open(base + ".json") as fh:
+            meta = json.load(fh)
+        if meta.pop("holomorphic") != bool(self.holomorphic):
+            return False
+        (mr, er), (mi, ei) = meta.pop("ref_point")
+        with mpmath.workprec(meta["prec"]):
+            meta["ref_point"] = mpmath.mpc(mpmath.mpf((mr, er)), mpmath.mpf((mi, ei)))
+        if self.holomorphic:
+            meta["xr"] = {r[0]: (r[1], r[2], r[3]) for r in meta["xr"]}
+        else:
+            meta["xr"] = {r[0]: ((r[1], r[2]), (r[3], r[4])) for r in meta["xr"]}
+        meta["init_kwargs"] = dict(meta["init_kwargs"])
+        self._FP_params = meta
+        self._Zn_path = np.load(base + ".npy", allow_pickle=False)
+        return True
 
     def ref_point_matching(self):
         """Reuse the stored orbit when the precision, location, max_iter and
         model match (perturbation.py:211-253)."""
-        fp = self._FP_params
+        fp = self.FP_params
         if fp is None:
             return False
         with mpmath.workprec(self.prec):
@@ -143,7 +226,7 @@ class PerturbationFractal(Fractal):
             loc_ok = abs(drift / self.dx) ** 2 < 1.e6
         return (self.prec <= fp["prec"] + 10 and loc_ok
                 and fp["max_iter"] >= self.max_iter
-                and fp["init_kwargs"] == self.init_kwargs
+                and _without_dir(fp["init_kwargs"]) == _without_dir(self.init_kwargs)
                 and fp.get("xr") is not None
                 and fp["xr_detect"] == self.xr_detect_activated)
 
@@ -204,13 +287,12 @@ class PerturbationFractal(Fractal):
         """Shallow frames: a constant orbit at the critical point, order 100
         (perturbation.py:770-805)."""
         div_iter = 100
-        self._FP_params = {
+        self.save_ref_point({
             "ref_point": mpmath.mpc(crit), "prec": self.prec, "order": 100,
             "max_iter": self.max_iter, "FP_code": self.FP_code, "xr": {},
             "div_iter": div_iter, "init_kwargs": self.init_kwargs,
             "xr_detect": self.xr_detect_activated,
-        }
-        self._Zn_path = np.full([div_iter + 1], complex(crit), dtype=np.complex128)
+        }, np.full([div_iter + 1], complex(crit), dtype=np.complex128))
 
     def compute_FP_orbit(self, ref_point, order=None):
         """Arbitrary-precision orbit, wrapped at the cycle order
@@ -220,13 +302,12 @@ class PerturbationFractal(Fractal):
         if order is not None:
             ref_orbit_len = min(order, ref_orbit_len)
         div_iter, Zn_path, xr = self.FP_loop(ref_orbit_len, ref_point)
-        self._FP_params = {
+        self.save_ref_point({
             "ref_point": mpmath.mpc(ref_point), "prec": self.prec, "order": order,
             "max_iter": max_iter, "FP_code": self.FP_code, "ref_orbit_len": ref_orbit_len,
             "xr": xr, "div_iter": div_iter, "init_kwargs": self.init_kwargs,
             "xr_detect": self.xr_detect_activated,
-        }
-        self._Zn_path = Zn_path
+        }, Zn_path)
 
     def set_reference(self, ref_point, order, Zn_path, div_iter, xr):
         """Install an orbit computed elsewhere (a cache or another process)."""
@@ -332,6 +413,12 @@ class PerturbationFractal(Fractal):
             # derivative paths at dx * projection.scale (perturbation.py:459-467)
             with mpmath.workprec(self.prec):
                 fields["deriv_mant"], fields["deriv_exp"] = mpf_to_xr(self.dx * scale)
+        band = getattr(self.projection, "band_scale", 1)
+        if band != 1:
+            # a deep Expmap band: offsets relative to exp(h_band), which the
+            # pixel scale takes over
+            with mpmath.workprec(self.prec):
+                fields["dx_mant"], fields["dx_exp"] = mpf_to_xr(self.dx * band)
         arrays = {
             "Zn_path": pd["Zn_path"].view(np.float64),
             "xr_index": pd["xr_index"], "xr_mant": pd["xr_mant"], "xr_exp": pd["xr_exp"],
@@ -366,15 +453,13 @@ class PerturbationFractal(Fractal):
                 Z[row] *= mod
         return rc
 
-    def calc_raw_by_steps(self, calc_name, exp_zoom_step=None, keep_bands=False):
-        """An exponential map computed by bands of h, each with the BLA
-        validity radii and the derivative scale of its own |c| bound, as the
-        reference's save_expdb_by_steps / reset_bla_tree do
-        (core.py:891-963, perturbation.py:565-580).  Every band shares the
-        device orbit and BLA coefficients of the first (only the radii are
-        rebuilt) and runs its tiles in one launch.  Returns calc_raw's dict
-        (whole-frame arrays, row-major (y, x)); with keep_bands, also
-        "bands": [(CycleIndepArgs, chunk slices)]."""
+    def expmap_bands(self, calc_name, exp_zoom_step=None):
+        """The bands of h of an exponential map (save_expdb_by_steps,
+        core.py:891-963): yields (fields, arrays, chunk slices) per band, the
+        descriptor with the BLA validity radii and derivative scale of the
+        band's own |c| bound (reset_bla_tree, perturbation.py:565-580).  The
+        projection is set to the band while the caller holds it, so that its
+        f and dzndc_modifier are the band's."""
         proj = self.projection
         if not isinstance(proj, fsproj.Expmap):
             raise ValueError("steps apply to the Expmap projection")
@@ -382,17 +467,9 @@ class PerturbationFractal(Fractal):
         state = cd["state"]
         if getattr(state, "subset", None) is not None:
             raise NotImplementedError("steps with a subset")
-        base = cd["cycle_indep_args"]
         nh = proj.nh(self)
         stp = int(exp_zoom_step or proj.nt(self))
         cs = settings.chunk_size
-        nx, ny = self.nx, self.ny
-        n_Z, n_U = len(state.codes[0]), len(state.codes[1])
-        Z = np.zeros((n_Z, ny, nx), dtype=state.complex_type)
-        U = np.zeros((n_U, ny, nx), dtype=self.int_type)
-        sr = -np.ones((1, ny, nx), dtype=self.termination_type)
-        si = np.zeros((1, ny, nx), dtype=self.int_type)
-        bands = []
         try:
             for r in range(0, nh + 1, stp):
                 i_max = min(r + stp, nh)
@@ -404,24 +481,44 @@ class PerturbationFractal(Fractal):
                 if not chunks:
                     continue
                 fields, arrays = self.get_ref_desc(cd["iterate"])
-                args = CycleIndepArgs(fields, arrays, self._interrupted, projection=proj).shared_with(base.ctx)
-                args.step_h = (proj.exp_step_hmax, proj.exp_step_hmin)
-                c_pix = np.concatenate([np.ravel(self.chunk_pixel_pos(c)) for c in chunks])
-                res = self._result_arrays(state, c_pix)
-                if self.numba_cycle_call(res, args) != 0:
-                    raise RuntimeError("calculation interrupted")
-                o = 0
-                for ix, ixx, iy, iyy in chunks:
-                    k = (ixx - ix) * (iyy - iy)
-                    for dst, src in zip((Z, U, sr, si), res[1:]):
-                        dst[:, iy:iyy, ix:ixx] = src[:, o:o + k].reshape(-1, iyy - iy, ixx - ix)
-                    o += k
-                if keep_bands:
-                    bands.append((args, chunks))
-                else:
-                    args.close()
+                yield fields, arrays, chunks
         finally:
             proj.del_exp_zoom_step()
+
+    def calc_raw_by_steps(self, calc_name, exp_zoom_step=None, keep_bands=False):
+        """An exponential map computed by bands of h (expmap_bands).  Every
+        band shares the device orbit and BLA coefficients of the first (only
+        the radii are rebuilt) and runs its tiles in one launch.  Returns
+        calc_raw's dict (whole-frame arrays, row-major (y, x)); with
+        keep_bands, also "bands": [(CycleIndepArgs, chunk slices)]."""
+        proj = self.projection
+        cd = self._calc_data[calc_name]
+        state = cd["state"]
+        base = cd["cycle_indep_args"]
+        nx, ny = self.nx, self.ny
+        n_Z, n_U = len(state.codes[0]), len(state.codes[1])
+        Z = np.zeros((n_Z, ny, nx), dtype=state.complex_type)
+        U = np.zeros((n_U, ny, nx), dtype=self.int_type)
+        sr = -np.ones((1, ny, nx), dtype=self.termination_type)
+        si = np.zeros((1, ny, nx), dtype=self.int_type)
+        bands = []
+        for fields, arrays, chunks in self.expmap_bands(calc_name, exp_zoom_step):
+            args = CycleIndepArgs(fields, arrays, self._interrupted, projection=proj).shared_with(base.ctx)
+            args.step_h = (proj.exp_step_hmax, proj.exp_step_hmin)
+            c_pix = np.concatenate([np.ravel(self.chunk_pixel_pos(c)) for c in chunks])
+            res = self._result_arrays(state, c_pix)
+            if self.numba_cycle_call(res, args) != 0:
+                raise RuntimeError("calculation interrupted")
+            o = 0
+            for ix, ixx, iy, iyy in chunks:
+                k = (ixx - ix) * (iyy - iy)
+                for dst, src in zip((Z, U, sr, si), res[1:]):
+                    dst[:, iy:iyy, ix:ixx] = src[:, o:o + k].reshape(-1, iyy - iy, ixx - ix)
+                o += k
+            if keep_bands:
+                bands.append((args, chunks))
+            else:
+                args.close()
         out = {"c_pix": np.ravel(self.chunk_pixel_pos((0, nx, 0, ny))),
                "Z": Z.reshape(n_Z, -1), "U": U.reshape(n_U, -1), "stop_reason": sr.reshape(1, -1),
                "stop_iter": si.reshape(1, -1), "codes": state.codes}
@@ -430,6 +527,11 @@ class PerturbationFractal(Fractal):
         return out
 
     def calc_raw(self, calc_name, jitter=False, supersampling=None):
+        if getattr(self.projection, "deep", False):
+            # a deep exponential map exists only by bands of h
+            if jitter or supersampling:
+                raise NotImplementedError("deep Expmap with jitter or supersampling")
+            return self.calc_raw_by_steps(calc_name)
         args = self._calc_data[calc_name]["cycle_indep_args"]
         res = super().calc_raw(calc_name, jitter, supersampling)
         if res is not None:

@@ -77,15 +77,22 @@ class Cartesian(Projection):
 class Expmap(Projection):
     """Exponential map z_pix -> exp(h_moy + dh z_pix), h from hmin to hmax
     along the screen's x axis ("horizontal") or y axis ("vertical")
-    (projection.py:226-500, without the stepwise mode of movie databases).
-    h is kept in binary64, so exp(hmax) must stay below 1e300 (the
-    reference switches h to Xrange above that)."""
+    (projection.py:226-500).
+
+    A deep map (exp(hmax) above 1e300: the reference holds h as Xrange
+    there, projection.py:287-294) spans more than binary64 can hold as a
+    pixel offset in units of dx.  It is computed by bands of h
+    (PerturbationFractal.calc_raw_by_steps; calc_raw does so by itself): in
+    a band, the offsets are exp(h - h_band + i t), within binary64, and the
+    band's descriptor takes dx exp(h_band) as its (extended-range) pixel
+    scale, which is also the derivative scale of the band
+    (perturbation.py:459-467): the same c = dx exp(h + i t) for every pixel."""
 
     def __init__(self, hmin, hmax, rotates_df=True, orientation="horizontal"):
         if not 0 <= hmin < hmax:
             raise ValueError("Provide hmin, hmax with:  0 <= hmin < hmax for Expmap")
-        if mpmath.exp(hmax) > 1. / settings.xrange_zoom_level:
-            raise NotImplementedError("Expmap with exp(hmax) above 1e300 (Xrange h)")
+        # h itself stays within binary64 (h ~ 4605 at 1e2000); only exp(h) does not
+        self.deep = bool(mpmath.exp(hmax) > 1. / settings.xrange_zoom_level)
         self.hmin = hmin
         self.hmax = hmax
         self.rotates_df = rotates_df
@@ -99,7 +106,8 @@ class Expmap(Projection):
     def set_exp_zoom_step(self, exp_step_hmax, exp_step_hmin):
         """Compute the next band of h only (projection.py:311-331): the
         derivative scale, the modifier's shift and the bounding box follow
-        the band; f does not change."""
+        the band; f changes only for a deep map (offsets relative to the
+        band)."""
         self.exp_step_hmax = float(exp_step_hmax)
         self.exp_step_hmin = float(exp_step_hmin)
         self.exp_step_hmoy = 0.5 * (self.exp_step_hmax + self.exp_step_hmin)
@@ -135,8 +143,24 @@ class Expmap(Projection):
             fractal.zoom_kwargs["nx"] = nx
 
     def f(self, pix):
-        """exp(h_moy + pix_to_ht * pix) (projection.py:363-372)."""
-        return np.exp(self.hmoy + self.pix_to_ht * np.asarray(pix, dtype=np.complex128))
+        """exp(h_moy + pix_to_ht * pix) (projection.py:363-372); for a deep
+        map, relative to the current band: exp(h_moy - h_band + pix_to_ht * pix)
+        (the band's pixel scale carries exp(h_band))."""
+        pix = np.asarray(pix, dtype=np.complex128)
+        if self.deep:
+            if not self.use_step:
+                raise ValueError("a deep Expmap (exp(hmax) > 1e300) is computed by bands of h "
+                                 "(calc_raw_by_steps)")
+            return np.exp((self.hmoy - self.exp_step_hmoy) + self.pix_to_ht * pix)
+        return np.exp(self.hmoy + self.pix_to_ht * pix)
+
+    @property
+    def band_scale(self):
+        """exp(h_band) of a deep map's current band, the factor between the
+        offsets f returns and the frame's dx (1 otherwise)."""
+        if self.deep and self.use_step:
+            return mpmath.exp(self.exp_step_hmoy)
+        return 1
 
     def dzndc_modifier(self, cpix):
         """exp(Re(pix_to_ht * cpix) + h_shift), h_shift = h_moy, or

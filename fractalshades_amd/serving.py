@@ -198,7 +198,7 @@ def _nuclei(n_centres, digits, seed0=100):
             continue
         prec = mpmath.libmp.dps_to_prec(digits)
         with mpmath.workprec(prec):
-            f = models.Perturbation_mandelbrot("/tmp/fractalshades_amd")
+            f = models.Perturbation_mandelbrot(None)
             f.prec = prec
             # convergence is |Newton step| <= 64 ulp; |z_order| at the
             # nucleus is then ~|dz/dc| ulp, so the validity bound is loose
@@ -234,7 +234,7 @@ def serve_centres(n_centres=8, digits=2100, cache=True):
     for nuc, order in nuclei:
         orbits = {}
         for xr_mode in (False, True):
-            f = models.Perturbation_mandelbrot("/tmp/fractalshades_amd")
+            f = models.Perturbation_mandelbrot(None)
             f.prec = prec
             f.M_divergence = 1.e3
             f.dx = mpmath.mpf("1e-400") if xr_mode else mpmath.mpf("1e-100")
@@ -245,6 +245,30 @@ def serve_centres(n_centres=8, digits=2100, cache=True):
     return centres
 
 
+def boundary_centre(ref_len=200001):
+    """A centre on the boundary of the set rather than at a nucleus: the
+    seeded 1e-2000 benchmark location (fractalshades_amd.configs, C2_2000:
+    a point brought within 1e-2000 of the boundary by the potential-gradient
+    descent, given to 2045 digits), with its orbit computed unwrapped from
+    that point (no cycle; it escapes after ~7,200 steps).  Frames around it
+    at any width down to 1e-2000 are exterior-dominated: their pixels escape
+    after a number of iterations that grows with the depth, instead of
+    stopping on the interior test of a nucleus's component."""
+    from . import configs, models
+    x, y, _, _ = configs.center_for("C2_2000", compute=False)
+    digits = configs.WORKLOADS["C2_2000"].digits
+    prec = mpmath.libmp.dps_to_prec(digits)
+    with mpmath.workprec(prec):
+        c = mpmath.mpc(x, y)
+    f = models.Perturbation_mandelbrot(None)
+    f.prec = prec
+    f.M_divergence = 1.e3
+    f.dx = mpmath.mpf("1e-400")
+    with mpmath.workprec(prec):
+        div_iter, Zn_path, xr = f.FP_loop(ref_len, c)
+    return Centre(c, None, digits, {True: (Zn_path, div_iter, xr), False: (Zn_path, div_iter, {})})
+
+
 def movie_centre(max_iter, centre=0, digits=2100):
     """The zoom sequence's centre: a cached nucleus with its orbit computed
     unwrapped up to max_iter (one resident orbit of max_iter + 1 points for
@@ -252,15 +276,14 @@ def movie_centre(max_iter, centre=0, digits=2100):
     from . import models
     nuc, _ = _cached_nuclei(digits)[centre]
     prec = mpmath.libmp.dps_to_prec(digits)
-    orbits = {}
-    for xr_mode in (False, True):
-        f = models.Perturbation_mandelbrot("/tmp/fractalshades_amd")
-        f.prec = prec
-        f.M_divergence = 1.e3
-        f.dx = mpmath.mpf("1e-400") if xr_mode else mpmath.mpf("1e-100")
-        with mpmath.workprec(prec):
-            div_iter, Zn_path, xr = f.FP_loop(max_iter + 1, nuc)
-        orbits[xr_mode] = (Zn_path, div_iter, xr)
+    f = models.Perturbation_mandelbrot(None)
+    f.prec = prec
+    f.M_divergence = 1.e3
+    f.dx = mpmath.mpf("1e-400")             # keeps the Xrange register
+    with mpmath.workprec(prec):
+        div_iter, Zn_path, xr = f.FP_loop(max_iter + 1, nuc)
+    # the binary64 frames read the same rounded orbit without the register
+    orbits = {True: (Zn_path, div_iter, xr), False: (Zn_path, div_iter, {})}
     return Centre(nuc, None, digits, orbits)
 
 
@@ -315,8 +338,15 @@ def serve_requests(n_requests=512, n_centres=8, seed=0, log10_width=(-20, -2000)
 
 def build_request(spec, centres, precision="fp64"):
     """Model + calculation of one request on its centre's resident orbit.
-    Returns (model, CycleIndepArgs, c_pix)."""
-    from . import models
+    Returns (model, CycleIndepArgs, c_pix); the CycleIndepArgs carries
+    ``resident_key``, the identity of the reference orbit it runs on, so
+    that requests with the same key can share one device copy of it.
+
+    Frames wider than settings.newton_zoom_level are computed around the
+    critical orbit with BLA off, as the reference computes every shallow
+    frame (perturbation.py:669-672, BLA_activated in mandelbrot_M2.py:521-524),
+    not around the centre's nucleus."""
+    from . import models, settings
     from .perturbation import PRECISIONS
     c = centres[spec.centre]
     width = mpmath.mpf(spec.width)
@@ -327,16 +357,21 @@ def build_request(spec, centres, precision="fp64"):
         x = c.nucleus.real + mpmath.mpf(spec.ox) * width
         y = c.nucleus.imag + mpmath.mpf(spec.oy) * width
         xs, ys = mpmath.nstr(x, digits + 5), mpmath.nstr(y, digits + 5)
-    f = models.Perturbation_mandelbrot("/tmp/fractalshades_amd")
+    f = models.Perturbation_mandelbrot(None)
     f.zoom(precision=digits, x=xs, y=ys, dx=spec.width, nx=spec.nx, xy_ratio=spec.nx / spec.ny, theta_deg=0.)
     f.max_iter = spec.max_iter
-    Zn_path, div_iter, xr = c.orbits[bool(f.xr_detect_activated)]
-    f.set_reference(c.nucleus, c.order, Zn_path, div_iter, xr)
+    shallow = width > settings.newton_zoom_level
+    if shallow:
+        f.get_FP_orbit()                      # the critical orbit
+    else:
+        Zn_path, div_iter, xr = c.orbits[bool(f.xr_detect_activated)]
+        f.set_reference(c.nucleus, c.order, Zn_path, div_iter, xr)
     f.calc_std_div(calc_name="serve", subset=None, max_iter=spec.max_iter, M_divergence=1.e3,
                    epsilon_stationnary=1.e-3, interior_detect=True, calc_dzndc=True, BLA_eps=1.e-6)
     cia = f._calc_data["serve"]["cycle_indep_args"]
     if PRECISIONS[precision] != cia.fields["precision"]:
         cia = cia.with_fields(precision=PRECISIONS[precision])
+    cia.resident_key = ("critical" if shallow else spec.centre, cia.fields["xr_mode"])
     c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, spec.nx, 0, spec.ny))))
     return f, cia, c_pix
 

@@ -184,3 +184,92 @@ def test_server_layers_match_host_postproc(name, model_kw):
     np.testing.assert_allclose(t.normal[esc, 0], nrm_ref.real.astype(np.float32), rtol=0, atol=1.2e-7)
     np.testing.assert_allclose(t.normal[esc, 1], nrm_ref.imag.astype(np.float32), rtol=0, atol=1.2e-7)
     t.close()
+
+
+def test_server_wait_from_several_threads():
+    """Two threads waiting on one ticket both get the request's status and
+    times; the ticket is released after them, so a later wait fails cleanly
+    (no access to a freed request)."""
+    import threading
+    f = frames.make("M2_E20", nx=96)
+    a, c = frames.frame_inputs(f)
+    ref = run_gpu(a, c)[0]
+    srv = TileServer(lanes=2, chunk_px=1024, hbm_budget=64 << 20)
+    try:
+        t = _tile_of(a, c)
+        tk = srv.submit(a.ctx, t)
+        got = [None, None]
+
+        def waiter(k):
+            got[k] = srv.wait(tk, 60000)
+
+        ths = [threading.Thread(target=waiter, args=(k,)) for k in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join(60)
+        assert all(g is not None and g["status"] == 0 for g in got)
+        assert got[0]["t_done_ms"] == got[1]["t_done_ms"]
+        with pytest.raises(FsgError):
+            srv.wait(tk, 1000)
+        assert_same((t.Z, t.U, t.stop_reason, t.stop_iter), ref)
+    finally:
+        srv.close()
+
+
+def test_serving_stream_subset():
+    """The seeded request stream of the serving benchmark (512 requests on
+    eight resident 2100-digit nuclei: sizes 256^2, 512^2, 1024^2, 4096x256;
+    widths log-uniform in [1e-2000, 1e-20]; max_iter log-uniform in
+    [1e3, 1e7]), a seeded subset of six requests of each size, plus every
+    other one in the binary32 form: all submitted at once to one server
+    (continuous batching across requests of different depth, size and
+    arithmetic), each result bit-identical to the same request run alone
+    (fsg_tile), and a strided subsample of each binary64 result bit-exact
+    with the oracle."""
+    import oracle as orc
+    from fractalshades_amd import serving
+    centres = serving.serve_centres()
+    specs = serving.serve_requests(512, len(centres), seed=0)
+    rng = np.random.default_rng(7)
+    pick = []
+    for size in serving.SIZES:
+        idx = [k for k, sp in enumerate(specs) if (sp.nx, sp.ny) == size]
+        pick += list(rng.choice(idx, 6, replace=False))
+    chosen = [specs[k] for k in pick]
+    assert min(int(sp.width.split("e")[1]) for sp in chosen) < -1500
+    assert max(sp.max_iter for sp in chosen) > 1e6
+    bases, reqs = {}, []
+    for k, sp in enumerate(chosen):
+        precision = "fp32x" if k % 2 else "fp64"
+        f, cia, c_pix = serving.build_request(sp, centres, precision)
+        key = (cia.resident_key, precision)
+        if key not in bases:
+            bases[key] = cia.ctx
+        else:
+            cia = cia.shared_with(bases[key])
+        reqs.append((sp, precision, cia, c_pix))
+    srv = TileServer(lanes=3, chunk_px=65536, hbm_budget=4 << 30)
+    try:
+        tiles, tickets = [], []
+        for sp, precision, cia, c_pix in reqs:
+            t = _tile_of(cia, c_pix)
+            tiles.append(t)
+            tickets.append(srv.submit(cia.ctx, t))
+        infos = [srv.wait(tk, 300000) for tk in tickets]
+    finally:
+        srv.close()
+    for (sp, precision, cia, c_pix), t, info in zip(reqs, tiles, infos):
+        print(f"stream {sp.nx}x{sp.ny} width {sp.width} max_iter {sp.max_iter} {precision} "
+              f"chunks {info['chunks']}", flush=True)
+        assert info["status"] == 0
+        alone, _ = run_gpu(cia, c_pix)
+        assert_same((t.Z, t.U, t.stop_reason, t.stop_iter), alone)
+        if precision == "fp64":
+            sel = np.arange(0, c_pix.size, 4099)
+            ref = orc.OracleCtx(cia.fields, cia.arrays).cycles(c_pix[sel])
+            assert_same(tuple(np.ascontiguousarray(a[..., sel]) for a in (t.Z, t.U, t.stop_reason, t.stop_iter)),
+                        ref)
+        t.close()
+    for _, _, cia, _ in reqs:
+        cia.close()

@@ -205,3 +205,48 @@ def test_expmap_vs_brute_force(orientation):
     unit = orc.OracleCtx(dict(args.fields, bla_on=0, deriv_mant=0., deriv_exp=0), args.arrays).cycles(cp)
     esc = off[2][0] == 1
     np.testing.assert_allclose(off[0][1, esc], unit[0][1, esc] * scale, rtol=1e-12)
+
+
+def test_deep_expmap_vs_brute_force():
+    """A deep exponential map (h in [0, 758], exp(hmax) ~ 1e329 > 1e300: the
+    reference's Xrange h, projection.py:287-294) at dx = 1e-330 around an
+    exterior point, computed by bands of h: per band the offsets
+    exp(h - h_band + i t) and the pixel scale dx exp(h_band); the map spans
+    |c - centre| from 1e-330 (Xrange deltas, the centre's own escape count)
+    to 0.1 (other counts).  On two sampled pixels of every band, the band's
+    oracle run (BLA off) escapes at the brute-force iteration of
+    c = centre + dx exp(h + i t) computed exactly, z_n within 1e-6."""
+    f = frames.expmap_frame("M2_E20", nx=1200, hmin=0., hmax=758., dx="1e-330", precision=345,
+                            no_newton=True, max_iter=3000, x="-0.75", y="0.2")
+    p = f.projection
+    assert p.deep
+    with pytest.raises(ValueError):
+        p.f(np.zeros(2))
+    rng = np.random.default_rng(3)
+    checked = n_bands = 0
+    counts = set()
+    for k, (fields, arrays, chunks) in enumerate(f.expmap_bands("t")):
+        n_bands += 1
+        for q, (ix, ixx, iy, iyy) in enumerate(chunks[:1] * 2):
+            i, j = int(rng.integers(ix, ixx)), int(rng.integers(iy, iyy))
+            if q and ixx == f.nx:
+                i = f.nx - 1                              # the largest |c - centre| of the map
+            pix = np.ravel(f.chunk_pixel_pos((ix, ixx, iy, iyy)))[(j - iy) * (ixx - ix) + (i - ix)]
+            off = p.f(np.array([pix]))                    # the band's offset of this pixel
+            assert abs(off[0]) < 1e100
+            Zo, Uo, sro, sio = orc.OracleCtx(dict(fields, bla_on=0), arrays).cycles(off)
+            with mpmath.workprec(f.prec + 64):
+                ht = mpmath.mpf(p.hmoy) + mpmath.mpc(complex(p.pix_to_ht * np.complex128(pix)))
+                c = mpmath.mpc(f.x, f.y) + f.dx * mpmath.exp(ht)
+                z = mpmath.mpc(0)
+                for n in range(1, fields["max_iter"] + 1):
+                    z = z * z + c
+                    if z.real ** 2 + z.imag ** 2 > 1e6:
+                        break
+            assert n == sio[0, 0], (k, i, j, n, sio[0, 0])
+            if sro[0, 0] == 1:
+                assert abs(Zo[0, 0] - complex(z)) <= 1e-6 * abs(complex(z))
+            counts.add(int(n))
+            checked += 1
+    assert n_bands >= 5 and checked >= 10
+    assert len(counts) >= 2          # the map spans depths with different escape counts

@@ -197,3 +197,43 @@ def test_image_mask_from_tiles(tmp_path):
                                    rtol=0, atol=1e-15)
     np.testing.assert_array_equal(f._image_mask(TileSubset()), np.ravel(pattern))
     np.testing.assert_array_equal(f._image_mask(pattern.ravel()), np.ravel(pattern))
+
+
+def test_reference_orbit_is_stored_and_reused(tmp_path, monkeypatch):
+    """The reference orbit is stored in <directory>/data/ref_pt.{json,npy}
+    (the reference's ref_pt.dat, perturbation.py:256-290, without pickle) and
+    reused by a later calculation at the same location with the same or a
+    smaller max_iter and precision (ref_point_matching, perturbation.py:211-253):
+    the same orbit, register and parameters, and no orbit is computed.  A
+    larger max_iter computes a new one."""
+    import fractalshades_amd as fsa
+    from fractalshades_amd import models
+
+    def frame(max_iter):
+        f = fsa.Perturbation_mandelbrot(str(tmp_path))
+        f.zoom(precision=360, x="-1.74920463345912691", y="-0.00028684660237361114", dx="1e-310",
+               nx=32, xy_ratio=1.0, theta_deg=0.)
+        f.no_newton = True
+        f.calc_std_div(calc_name="t", subset=None, max_iter=max_iter, M_divergence=1.e3,
+                       epsilon_stationnary=1.e-3, BLA_eps=1.e-6, interior_detect=True)
+        return f
+
+    a = frame(3000)
+    a.get_FP_orbit()
+    fp, Zn = a.FP_params, np.array(a.Zn_path)
+    assert os.path.exists(os.path.join(str(tmp_path), "data", "ref_pt.json"))
+    calls = []
+    orig = models.Perturbation_mandelbrot.FP_loop
+    monkeypatch.setattr(models.Perturbation_mandelbrot, "FP_loop",
+                        lambda self, *args: calls.append(args) or orig(self, *args))
+    b = frame(2000)
+    assert b.ref_point_matching()
+    b.get_FP_orbit()
+    assert calls == []
+    np.testing.assert_array_equal(b.Zn_path.view(np.uint64), Zn.view(np.uint64))
+    assert b.FP_params["xr"] == fp["xr"] and b.FP_params["div_iter"] == fp["div_iter"]
+    assert b.FP_params["ref_point"] == fp["ref_point"]
+    assert b.get_path_data()["xr_index"].tolist() == a.get_path_data()["xr_index"].tolist()
+    c = frame(4000)          # its calculation computes and stores a new orbit
+    assert len(calls) == 1
+    assert c.FP_params["max_iter"] == 4000 and c.Zn_path.size == 4001
