@@ -130,14 +130,18 @@ FSG_DEV bool le(xlf a, xlf b) {
 
 // Conversion to binary64 with the reference's procedure (to_standard,
 // numba_xr.py:802-829: per-component frexp, co-exponent, product by
-// 2^exp).  For a normalised value whose smaller component is zero or a
-// normal number the co-exponent shift is exact and the procedure reduces to
-// two products by 2^e; otherwise it is run as written.
+// 2^exp in complex arithmetic).  For a normalised value whose smaller
+// component is zero or a normal number the co-exponent shift is exact and
+// the procedure reduces to two products by 2^e, provided neither product is
+// zero (numba_cscale sets the sign of a zero); otherwise it is run as
+// written.
 FSG_DEV cplx to_std(xl a) {
     const double mn = fmin(fabs(a.re), fabs(a.im));
     if (mn != 0. && mn < 0x1p-1022) return to_std(xc_of(a));
     const double s = ldexp(1., max(a.e, -1100));
-    return mk(a.re * s, a.im * s);
+    const cplx p = mk(a.re * s, a.im * s);
+    if (p.re != 0. && p.im != 0.) return p;
+    return numba_cscale(a.re, a.im, s);
 }
 FSG_DEV double to_std(xlf a) { return ldexp(a.m, max(a.e, -2200)); }
 

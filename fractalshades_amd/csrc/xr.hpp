@@ -118,12 +118,22 @@ FSG_HD xc norm_c(double re, double im, int32_t e) {
 FSG_HD xc to_xc(cplx z) { return norm_c(z.re, z.im, 0); }
 FSG_HD xf to_xf(double x) { return norm_f(x, 0); }
 FSG_HD double to_std(xf a) { return ldexp_(a.m, a.e); }
+// (x + 1j * y) * s as numba evaluates it (to_standard, numba_xr.py:825-828):
+// every float operand is promoted to a complex (v + 0j) and the products and
+// sums are complex ones, (a+bi)(c+di) = (ac - bd) + (ad + bc)i.  The result
+// is (x s, y s) unless a component is zero; the 0 * y, +0 and im * 0 terms
+// then fix the sign of that zero.
+FSG_HD cplx numba_cscale(double x, double y, double s) {
+    const double re1 = 0. * y - 1. * 0.;           // (0 + 1j) * (y + 0j)
+    const double im1 = 0. * 0. + 1. * y;
+    const double re2 = x + re1, im2 = 0. + im1;    // (x + 0j) + ...
+    return mk(re2 * s - im2 * 0., re2 * 0. + im2 * s);
+}
 FSG_HD cplx to_std(xc a) {
     xf r = norm_f(a.re, a.e), i = norm_f(a.im, a.e);
     int32_t oe;
     coexp(r.m, r.e, i.m, i.e, oe);
-    double s = ldexp_(1., oe);
-    return mk(r.m * s, i.m * s);
+    return numba_cscale(r.m, i.m, ldexp_(1., oe));
 }
 
 // ---- arithmetic: complex ----

@@ -162,6 +162,19 @@ static inline xrc normalize_cplx(cpx m, int32_t e) {
 static inline xrc to_xrc(cpx z) { return normalize_cplx(z, 0); }
 static inline xrf to_xrf(double x) { return normalize_real(x, 0); }
 
+/* (x + 1j * y) * s as numba evaluates it: each float operand is promoted to
+ * a complex (v + 0j) and the complex product and sum are taken as written,
+ * (a+bi)(c+di) = (ac - bd) + (ad + bc)i.  The result equals (x s, y s)
+ * except where a component is zero: 1j * y contributes a 0 * y term, the
+ * promotion adds +0 to the imaginary part, and the final product adds
+ * -(im * 0) and re * 0, which fixes the signs of zero results. */
+static inline cpx numba_cscale(double x, double y, double s) {
+    double re1 = 0. * y - 1. * 0.;          /* (0 + 1j) * (y + 0j) */
+    double im1 = 0. * 0. + 1. * y;
+    double re2 = x + re1, im2 = 0. + im1;   /* (x + 0j) + ... */
+    return C(re2 * s - im2 * 0., re2 * 0. + im2 * s);   /* * (s + 0j) */
+}
+
 /* to_standard, numba_xr.py:802-829 */
 static inline double xrf_std(xrf a) { return ldexp(a.m, a.e); }
 static inline cpx xrc_std(xrc a) {
@@ -169,8 +182,7 @@ static inline cpx xrc_std(xrc a) {
     xrf i = normalize_real(a.m.im, a.e);
     double o0, o1; int32_t oe;
     coexp_real(r.m, r.e, i.m, i.e, &o0, &o1, &oe);
-    double s = ldexp(1., oe);
-    return C(o0 * s, o1 * s);
+    return numba_cscale(o0, o1, ldexp(1., oe));
 }
 
 /* add / sub: numba_xr.py:318-388 */

@@ -75,6 +75,41 @@ def test_to_standard_exact():
     np.testing.assert_array_equal(res, std)
 
 
+def test_to_standard_signed_zeros():
+    """to_standard of a complex (numba_xr.py:825-828) evaluates
+    (co_nm_real + 1j * co_nm_imag) * ldexp(1., co_nexp) with the float
+    operands promoted to complex, so the sign of a zero component (an exact
+    zero, or a product that underflows) follows complex arithmetic, not a
+    per-component product.  CPython 3.10 promotes mixed float / complex
+    operands the same way, so the expression evaluated by Python is the
+    check.  Inputs are normalised (larger component in [1, 2), the other
+    within 2^-60 of it, or a zero of either sign), where the co-exponent
+    step leaves the mantissas as they are."""
+    import sys
+    assert sys.version_info < (3, 14)    # 3.14 adds mixed-mode complex arithmetic
+    rg = np.random.default_rng(11)
+    n = 4000
+    big = (1. + rg.random(n)) * rg.choice([-1., 1.], n)
+    small = ldexp_arr((1. + rg.random(n)) * rg.choice([-1., 1.], n), -rg.integers(0, 60, n))
+    small[rg.random(n) < 0.3] = 0.
+    small[rg.random(n) < 0.15] = -0.
+    swap = rg.random(n) < 0.5
+    re, im = np.where(swap, small, big), np.where(swap, big, small)
+    # half of the exponents where the products underflow to zero
+    e = np.where(rg.random(n) < 0.5, rg.integers(-1200, -1070, n), rg.integers(-1000, 60, n)).astype(np.int32)
+    got = orc.xrc_to_std(np.stack([re, im], 1).ravel().copy(), e).view(np.complex128)
+    want = np.array([(float(x) + 1j * float(y)) * math.ldexp(1., int(k)) for x, y, k in zip(re, im, e)])
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    # the cases where the promoted product differs from the per-component one occur
+    naive = np.array([complex(float(x) * math.ldexp(1., int(k)), float(y) * math.ldexp(1., int(k)))
+                      for x, y, k in zip(re, im, e)])
+    assert np.count_nonzero(naive.view(np.uint64) != want.view(np.uint64)) > 50
+
+
+def ldexp_arr(m, k):
+    return np.array([math.ldexp(float(a), int(b)) for a, b in zip(m, k)])
+
+
 def test_normalize_exact():
     m, e, std = gen_xr(5000, max_bin_exp=200)
     om, oe = orc.xrc_normalize(m.view(np.float64).copy(), e)
