@@ -77,6 +77,7 @@ class Stats(ctypes.Structure):
         ("plain_steps", ctypes.c_uint64), ("bla_steps", ctypes.c_uint64),
         ("bla_iters", ctypes.c_uint64), ("rebases", ctypes.c_uint64),
         ("bla_lookups", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
+        ("redo_pixels", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -200,7 +201,7 @@ def load(path=None):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.fsg_abi_version() != 3:
+        if lib.fsg_abi_version() != 4:
             raise FsgError("libfsgpu.so ABI version mismatch")
         if path is None:
             _lib = lib

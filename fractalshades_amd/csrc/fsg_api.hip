@@ -12,6 +12,8 @@
 #include <vector>
 #include <cstring>
 #include <climits>
+#include <chrono>
+#include <thread>
 
 using namespace fsg;
 
@@ -31,6 +33,12 @@ int fail(int code, const std::string &msg) { g_err = msg; return code; }
 __global__ void k_zero_u64(unsigned long long *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// stop_reason = -1 (not computed), the reference's initial value (core.py:2072)
+__global__ void k_fill_i8(int8_t *p, int64_t n, int8_t v) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 constexpr int WORK_WORDS = 8;
@@ -209,6 +217,10 @@ struct fsg_ctx {
     std::mutex mu;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // interrupt flag the tile kernel reads at each refill: pinned host
+    // memory, written by the host while it waits for a tile
+    uint32_t *stop_h = nullptr;
+    const uint32_t *stop_d = nullptr;
     char *buf = nullptr;
     size_t buf_bytes = 0;
     unsigned long long *work = nullptr;
@@ -227,6 +239,7 @@ struct fsg_ctx {
         if (buf) (void)hipFree(buf);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (stop_h) (void)hipHostFree(stop_h);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -351,17 +364,28 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8_t *sr, int32_t *si,
            unsigned long long *work, hipStream_t stream, bool timed, int64_t ld = -1, int max_blocks = 0,
-           int64_t grid_w = 0) {
+           int64_t grid_w = 0, const uint32_t *stop = nullptr) {
     KArgs a;
     fill_args(c, a);
     a.pix = pix; a.npts = n; a.ld = ld < 0 ? n : ld; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si;
-    a.work = work; a.max_blocks = max_blocks; a.grid_w = grid_w;
+    a.work = work; a.max_blocks = max_blocks; a.grid_w = grid_w; a.stop = stop;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, work, WORK_WORDS);
     FSG_HIP(hipGetLastError());
     if (n == 0) return 0;
     if (timed) FSG_HIP(hipEventRecord(c->ev0, stream));
-    if (c->d.precision == FSG_PREC_FP32X) FSG_HIP(launch_tile_f32(cfg_of(c), a, stream));
-    else FSG_HIP(launch_tile(cfg_of(c), a, stream));
+    if (c->d.precision == FSG_PREC_FP32X) {
+        FSG_HIP(launch_tile_f32(cfg_of(c), a, stream));
+        if (c->d.kind != FSG_KIND_BS) {
+            // the pixels the binary32 pass left unsure, again in binary64
+            KArgs r = a;
+            r.redo = 1;
+            LaunchCfg cf = cfg_of(c);
+            cf.precision = FSG_PREC_FP64;
+            FSG_HIP(launch_tile(cf, r, stream));
+        }
+    } else {
+        FSG_HIP(launch_tile(cfg_of(c), a, stream));
+    }
     if (timed) FSG_HIP(hipEventRecord(c->ev1, stream));
     return 0;
 }
@@ -400,10 +424,33 @@ int tile_staged(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, doub
     const size_t off_z = align256(pb), off_u = off_z + align256(zb), off_i = off_u + align256(ub);
     const size_t off_r = off_i + align256(ub);
     if (n > 0) FSG_HIP(hipMemcpyAsync(b, c_pix, pb, hipMemcpyHostToDevice, c->stream));
+    const uint32_t *stop = nullptr;
+    if (interrupted && n > 0) {
+        // an interrupted tile leaves its pixels not taken at stop_reason -1
+        *c->stop_h = 0u;
+        stop = c->stop_d;
+        hipLaunchKernelGGL(k_fill_i8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           reinterpret_cast<int8_t *>(b + off_r), n, (int8_t)-1);
+        FSG_HIP(hipGetLastError());
+    }
     int rc = launch(c, reinterpret_cast<double *>(b), n, reinterpret_cast<double *>(b + off_z),
                     reinterpret_cast<int32_t *>(b + off_u), reinterpret_cast<int8_t *>(b + off_r),
-                    reinterpret_cast<int32_t *>(b + off_i), work, c->stream, true, -1, 0, grid_w);
+                    reinterpret_cast<int32_t *>(b + off_i), work, c->stream, true, -1, 0, grid_w, stop);
     if (rc) return rc;
+    if (stop) {
+        // wait for the kernel (its end event), passing the caller's interrupt
+        // flag on to it (the reference polls it after every pixel,
+        // perturbation.py:1042-1043); the copies to pageable host memory are
+        // enqueued after, since enqueueing them can block this thread
+        int spins = 0;
+        hipError_t q;
+        while ((q = hipEventQuery(c->ev1)) == hipErrorNotReady) {
+            if (*interrupted) *c->stop_h = 1u;
+            if (++spins < 200) std::this_thread::yield();
+            else std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+        if (q != hipSuccess) return fail(FSG_E_HIP, std::string("tile: ") + hipGetErrorString(q));
+    }
     if (n > 0) {
         FSG_HIP(hipMemcpyAsync(Z, b + off_z, zb, hipMemcpyDeviceToHost, c->stream));
         FSG_HIP(hipMemcpyAsync(U, b + off_u, ub, hipMemcpyDeviceToHost, c->stream));
@@ -416,7 +463,7 @@ int tile_staged(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, doub
         unsigned long long w[WORK_WORDS];
         FSG_HIP(hipMemcpy(w, work, sizeof(w), hipMemcpyDeviceToHost));
         stats->plain_steps = w[1]; stats->bla_steps = w[2]; stats->bla_iters = w[3]; stats->rebases = w[4];
-        stats->bla_lookups = w[5];
+        stats->bla_lookups = w[5]; stats->redo_pixels = w[6];
         float ms = 0.f;
         if (n > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) stats->kernel_ms = ms;
     }
@@ -503,6 +550,16 @@ int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(FSG_E_HIP, "stream/event creation failed"));
+    {
+        void *hp = nullptr, *dp = nullptr;
+        if (hipHostMalloc(&hp, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return cleanup(fail(FSG_E_NOMEM, "hipHostMalloc interrupt flag"));
+        c->stop_h = static_cast<uint32_t *>(hp);
+        *c->stop_h = 0u;
+        if (hipHostGetDevicePointer(&dp, hp, 0) != hipSuccess)
+            return cleanup(fail(FSG_E_HIP, "hipHostGetDevicePointer interrupt flag"));
+        c->stop_d = static_cast<const uint32_t *>(dp);
+    }
 
     const int64_t L = c->L;
     const int32_t nx = desc->n_xr;
@@ -751,7 +808,7 @@ int fsg_work_stats(const void *d_work, fsg_stats *stats) {
     unsigned long long w[WORK_WORDS];
     FSG_HIP(hipMemcpy(w, d_work, sizeof(w), hipMemcpyDeviceToHost));
     stats->plain_steps = w[1]; stats->bla_steps = w[2]; stats->bla_iters = w[3]; stats->rebases = w[4];
-    stats->bla_lookups = w[5];
+    stats->bla_lookups = w[5]; stats->redo_pixels = w[6];
     return 0;
 }
 

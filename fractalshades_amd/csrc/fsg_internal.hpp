@@ -79,8 +79,15 @@ struct KArgs {
     LdsPlan lds;                      // lds.bytes == 0: tables read from global memory
     LdsPlanX ldsx;                    // the same for the binary32 form
     // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
-    // rebase, bla lookups)
+    // rebase, bla lookups), [6] pixels the binary32 form left unsure, [7] the
+    // cursor of the binary64 pass over them
     unsigned long long *work;
+    // 1: the binary64 pass over the pixels whose stop_reason the binary32
+    // pass set to REASON_UNSURE (every other pixel is skipped)
+    int32_t redo; int32_t pad3;
+    // host-visible stop flag (pinned host memory) or null: once it is
+    // nonzero, lanes take no new pixel; the pixels in flight finish
+    const uint32_t *stop;
 };
 
 struct LaunchCfg { int kind, degree, flavor, xr, precision, orbit; };
@@ -101,6 +108,7 @@ hipError_t launch_bla_build(int kind, int degree, int flavor, const cplx *Zn, in
                             hipStream_t stream, bool radii_only = false);
 int tile_kernel_blocks(const LaunchCfg &cfg);
 int grid_fill_blocks(const void *kernel);          // blocks filling the GPU at full occupancy
+int fill_blocks(const void *kernel, int block, int lds_bytes);   // the same for any block / LDS size
 // plan the LDS staging of a launch's tables (bytes = 0 when they do not fit)
 LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes);
 int launch_blocks(int fill, const KArgs &a, int block = 256);   // capped by max_blocks and the pixel count

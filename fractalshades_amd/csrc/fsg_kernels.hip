@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <atomic>
 #include <unordered_map>
 
 namespace fsg {
@@ -174,25 +175,38 @@ static tile_fn select_tile(const LaunchCfg &c) {
 }
 
 // blocks that fill the GPU at the kernel's occupancy, cached per kernel
-int grid_fill_blocks(const void *fn) {
+// Blocks of `block` lanes with `lds` bytes of dynamic LDS that fill the
+// current device at full occupancy, or -1.  Cached per (kernel, device,
+// block, LDS bytes) under a lock: the occupancy query costs far more than a
+// launch of a small serving chunk, and callers come from several threads.
+int fill_blocks(const void *fn, int block, int lds) {
+    struct Key {
+        const void *fn; int dev, block, lds;
+        bool operator==(const Key &o) const { return fn == o.fn && dev == o.dev && block == o.block && lds == o.lds; }
+    };
+    struct Hash {
+        size_t operator()(const Key &k) const {
+            return std::hash<const void *>()(k.fn) ^ ((size_t)k.dev << 48) ^ ((size_t)k.block << 32) ^ (size_t)k.lds;
+        }
+    };
     static std::mutex mu;
-    static std::unordered_map<long long, int> cache;
+    static std::unordered_map<Key, int, Hash> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -1;
-    const long long key = (long long)(uintptr_t)fn * 64 + dev;
-    {
-        std::lock_guard<std::mutex> g(mu);
-        auto it = cache.find(key);
-        if (it != cache.end()) return it->second;
-    }
+    const Key key{fn, dev, block, lds};
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
     int ncu = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) return -1;
-    const int blocks = ncu * (per_cu < 1 ? 1 : per_cu);
-    std::lock_guard<std::mutex> g(mu);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, (size_t)lds) != hipSuccess) return -1;
+    // without dynamic LDS one block per CU always fits; with it, none may
+    const int blocks = per_cu > 0 ? ncu * per_cu : (lds == 0 ? ncu : -1);
     cache[key] = blocks;
     return blocks;
 }
+
+int grid_fill_blocks(const void *fn) { return fill_blocks(fn, 256, 0); }
 
 int tile_kernel_blocks(const LaunchCfg &cfg) {
     tile_fn f = select_tile(cfg);
@@ -210,12 +224,16 @@ int launch_blocks(int fill, const KArgs &a, int block) {
 }
 
 int max_lds_bytes() {
-    static int cached = -1;
+    // per device, queried once (the first caller on a device stores it; a
+    // concurrent first caller stores the same value)
+    static std::atomic<int> per_dev[64] = {};
+    int dev = 0, cached = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    cached = per_dev[dev].load(std::memory_order_relaxed) - 1;
     if (cached < 0) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 0;
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 0;
+        per_dev[dev].store(v + 1, std::memory_order_relaxed);
         cached = v;
     }
     // FSG_LDS_BYTES caps the staging budget (to compare partial plans)
@@ -296,12 +314,9 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
         al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());
         tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg, lds_plan_full(a.t, al.lds)) : nullptr;
         const int block = tile_lds_block(cfg);
-        int per_cu = 0, dev = 0, ncu = 0;
-        if (f && hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(f), block,
-                                                         (size_t)al.lds.bytes) == hipSuccess && per_cu > 0) {
-            hipLaunchKernelGGL(f, dim3(launch_blocks(ncu * per_cu, al, block)), dim3(block), (size_t)al.lds.bytes,
+        const int fill = f ? fill_blocks(reinterpret_cast<const void *>(f), block, al.lds.bytes) : -1;
+        if (fill > 0) {
+            hipLaunchKernelGGL(f, dim3(launch_blocks(fill, al, block)), dim3(block), (size_t)al.lds.bytes,
                                stream, al);
             return hipGetLastError();
         }

@@ -10,6 +10,14 @@ namespace fsg {
 
 struct Counters { unsigned long long plain, bla, bla_it, reb, tries; };
 
+// Stop reason of a pixel whose binary32 result may differ from the binary64
+// loop's (a decision taken within the pixel's error bound): set by the
+// FSG_PREC_FP32X kernels, replaced by a second, binary64 pass over exactly
+// those pixels (KArgs::redo).  Never left in a finished result.
+constexpr int REASON_UNSURE = 3;
+// workspace words: [0] cursor, [1..5] counters, [6] unsure pixels, [7] the
+// cursor of the binary64 pass over them
+
 // --------------------------------------------------------------------------
 // Pixel order over a grid of rows of w pixels (n = h w + a short last row).
 // Slot s of the cursor maps to one pixel, a bijection of [0, n): first the
@@ -96,8 +104,10 @@ template <class Model>
 __device__ __forceinline__ void tile_loop(const KArgs &a) {
     typename Model::State s;
     Counters k = {0, 0, 0, 0, 0};
+    unsigned long long unsure = 0;
     int64_t idx = -1;
     int reason = 0;
+    unsigned long long *const cursor = a.work + (a.redo ? 7 : 0);
     bool active = false;      // iterating a pixel
     bool pending = false;     // holding a stopped pixel whose results are not written yet
     bool exhausted = false;   // the cursor has no pixel left for this lane
@@ -107,6 +117,7 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
             if (!active) {
                 if (pending) {
                     Model::finish(s, a, idx, reason);
+                    unsure += (reason == REASON_UNSURE);
                     pending = false;
                 }
                 if (!exhausted) {
@@ -116,23 +127,34 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
                     const unsigned rank = __builtin_amdgcn_mbcnt_hi(
                         (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                     unsigned long long base = 0;
-                    if ((int)__lane_id() == leader)
-                        base = atomicAdd(a.work, (unsigned long long)__popcll(mask));
+                    unsigned stop = 0;
+                    if ((int)__lane_id() == leader) {
+                        base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
+                        // the interrupt flag, read beside the cursor (its latency overlaps)
+                        if (a.stop) stop = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
                     const unsigned lo = __builtin_amdgcn_readlane((unsigned)base, leader);
                     const unsigned hi = __builtin_amdgcn_readlane((unsigned)(base >> 32), leader);
                     idx = (int64_t)(((unsigned long long)hi << 32) | lo) + rank;
-                    if (idx < a.npts) {
+                    if (__builtin_amdgcn_readlane(stop, leader) != 0u) {
+                        exhausted = true;          // interrupted: no new pixel
+                    } else if (idx < a.npts) {
                         if (a.grid_w > 0) idx = grid_slot(idx, a.npts, a.grid_w);
-                        Model::init(s, a, a.pix[2 * idx], a.pix[2 * idx + 1]);
-                        active = true;
+                        // the binary64 pass takes only the pixels the binary32 one left unsure
+                        if (!a.redo || a.stop_reason[idx] == REASON_UNSURE) {
+                            Model::init(s, a, a.pix[2 * idx], a.pix[2 * idx + 1]);
+                            active = true;
+                        }
                     } else {
                         exhausted = true;
                     }
                 }
             }
-            // every lane idle after the refill: the cursor is exhausted and
-            // every result has been written
-            if (__ballot(active) == 0ull) break;
+            // every lane idle after the refill: done once every lane has
+            // found the cursor exhausted (every result is then written); the
+            // binary64 pass can draw a whole batch of slots without an unsure
+            // pixel, and then refills again
+            if (__ballot(active) == 0ull && __ballot(!exhausted) == 0ull) break;
         }
         if (active) {
             reason = Model::step(s, a, k);
@@ -145,10 +167,12 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
     // all lanes have left the loop: one atomic per wave per counter
     unsigned long long c0 = wave_sum(k.plain), c1 = wave_sum(k.bla);
     unsigned long long c2 = wave_sum(k.bla_it), c3 = wave_sum(k.reb), c4 = wave_sum(k.tries);
+    unsigned long long c5 = wave_sum(unsure);
     if (__lane_id() == 0) {
         atomicAdd(a.work + 1, c0); atomicAdd(a.work + 2, c1);
         atomicAdd(a.work + 3, c2); atomicAdd(a.work + 4, c3);
         atomicAdd(a.work + 5, c4);
+        if (c5) atomicAdd(a.work + 6, c5);
     }
 }
 

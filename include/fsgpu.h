@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 3
+#define FSG_ABI_VERSION 4
 
 /* model kinds */
 #define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
@@ -103,6 +103,8 @@ typedef struct fsg_stats {
     uint64_t rebases;        /* rebases (reference divergence + dynamic glitch) */
     uint64_t bla_lookups;    /* BLA table lookups (attempts, accepted or not) */
     double kernel_ms;        /* device time of the tile kernel (HIP events) */
+    uint64_t redo_pixels;    /* FSG_PREC_FP32X: pixels recomputed in binary64 because a
+                                decision of the binary32 loop fell within its error bound */
 } fsg_stats;
 
 typedef struct fsg_ctx fsg_ctx;
@@ -149,7 +151,10 @@ int fsg_ctx_destroy(fsg_ctx *ctx);
  *   stop_reason [n] int8
  *   stop_iter   [n] int32
  *   interrupted optional pointer to the reference's _interrupted flag
- *               (core.py:1352); checked before and after the launch
+ *               (core.py:1352), polled while the tile runs: once it is set,
+ *               the kernel takes no new pixel (the pixels in flight finish)
+ *               and the pixels never taken keep stop_reason -1, their other
+ *               fields unspecified
  *   stats       optional
  * Synchronous.  Returns FSG_INTERRUPTED when *interrupted was set.
  */
@@ -307,7 +312,9 @@ int fsg_server_submit_layers(fsg_server *srv, fsg_ctx *ctx, const double *c_pix,
                              int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
 /* Wait for a request (timeout_ms < 0: no limit).  Returns its status, or
  * FSG_PENDING on timeout; after a return other than FSG_PENDING the ticket
- * is released. */
+ * is released.  Several threads may wait on one ticket at the same time:
+ * all of them return the request's status (and info); a wait that starts
+ * after the ticket was released fails with FSG_E_INVALID. */
 int fsg_server_wait(fsg_server *srv, uint64_t ticket, int32_t timeout_ms, fsg_request_info *info);
 
 /* Pinned host memory for request buffers. */
