@@ -286,6 +286,370 @@ struct HoloF32 {
 };
 
 // ---------------------------------------------------------------------------
+// Holomorphic models on scaled binary32 mantissas, with a running error
+// bound.
+//
+// z, dz/dc and dz/dz are each a pair of floats times 2^e, with one exponent
+// per quantity that stays fixed between renormalisations: the mantissas are
+// renormalised at every BLA point (w a multiple of 8, before the lookup),
+// after a BLA jump, at a rebase, and when one grows past 2^32.  In between,
+// a z^2 + c step is plain binary32 arithmetic on the mantissas (c is held in
+// z's scale), with fused multiply-adds; z's value in absolute terms, needed
+// next to the orbit point, is one ldexp.  The escape and rebase tests run in
+// binary64 on the binary64 orbit, as the drop-in's do.
+//
+// e2 estimates the variance of z - z_64 (in z's scale squared), z_64 being
+// the binary64 loop's delta at the same step: each step multiplies it by
+// |f'(Z + z)|^2 and adds the square of the step's own rounding (KE x the
+// magnitudes of the terms, c and the orbit point being rounded to binary32
+// too).  Rounding errors of successive steps are independent, so they add
+// in quadrature; a worst-case sum, linear in the step count, would exceed
+// the actual error by orders of magnitude after a thousand steps and leave
+// no decision certain.  The error bound is ERR_SIGMAS standard deviations.
+// The multiplier is the exact squared modulus: an over-estimate compounds
+// over thousands of steps while z itself grows by the modulus only.  Every decision the loop takes
+// on z -- the BLA stage (|z| against the radii), the escape, the rebase --
+// is checked against that bound: when the binary64 loop could have decided
+// otherwise, the pixel stops as REASON_UNSURE and a binary64 pass
+// recomputes it.  The interior test is checked against a relative margin.
+constexpr float KE = 4.f * 0x1p-24f;          // fresh error per step, x (term magnitudes)
+constexpr double ERR_SIGMAS = 8.;              // error bound, in standard deviations
+constexpr float RENORM_AT = 0x1p32f;           // mantissa 1-norm that forces a renormalisation
+constexpr double INTERIOR_MARGIN = 1e-3;       // relative margin of the stationarity test
+
+// BLA stage selection of bla_find with its certainty: |z| is known to
+// within eb, so the binary64 loop takes the same stage unless a radius lies
+// within eb of |z|; only the accepted stage's radius and the next stage's
+// can (the radii do not grow with the stage for one start).
+__device__ __forceinline__ int32_t bla_find_m(const Tables &t, double az, double eb, int32_t w,
+                                              int32_t &idx, bool &unsure) {
+    const int32_t it = w >> 3;
+    const int32_t invalid_step = t.first_invalid - w;
+    if (invalid_step <= 8) return 0;
+    const int32_t tz = (it == 0) ? 31 : __builtin_ctz((uint32_t)it);
+    const int32_t s_inv = 31 - __builtin_clz((uint32_t)(invalid_step - 1));
+    const int32_t cap = min(min(3 + tz, t.stages_bla - 1), s_inv);
+    int32_t hi = cap, lo = 3, best = 0;
+    double rb = 0.;
+    while (lo <= hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        const double r = t.r[2 * it + (1 << (mid - 3)) - 1];
+        if (az < r) { best = mid; rb = r; lo = mid + 1; }
+        else hi = mid - 1;
+    }
+    if (best > 0 && rb - az <= eb) unsure = true;
+    const int32_t nxt = best > 0 ? best + 1 : 3;
+    if (nxt <= cap && az - t.r[2 * it + (1 << (nxt - 3)) - 1] < eb) unsure = true;
+    if (best == 0) return 0;
+    idx = 2 * it + (1 << (best - 3)) - 1;
+    return 1 << best;
+}
+
+__device__ __forceinline__ float l1(float x, float y) { return fabsf(x) + fabsf(y); }
+__device__ __forceinline__ float l2(float x, float y) { return __builtin_sqrtf(x * x + y * y); }
+
+template <int KIND, int DEG>
+struct HoloS32 {
+    struct State { xs z, c, d, h; float cx, cy, e2; int32_t w, n; bool dyn, nul; };
+
+    // c in z's scale, after z's exponent moved
+    __device__ static void sync_c(State &s) {
+        const int32_t sc = min(s.c.e - s.z.e, 100);
+        s.cx = ldexpf(s.c.x, sc); s.cy = ldexpf(s.c.y, sc);
+    }
+    __device__ static void renorm_z(State &s) {
+        const float mz = fmaxf(fabsf(s.z.x), fabsf(s.z.y));
+        const int32_t k = (mz == 0.f) ? 0 : fexp(mz) - 1;
+        s.z.x = ldexpf(s.z.x, -k); s.z.y = ldexpf(s.z.y, -k); s.z.e += k;
+        s.e2 = ldexpf(s.e2, -2 * k);
+        sync_c(s);
+    }
+    __device__ static void renorm(State &s) {
+        renorm_z(s);
+        s.d = xs_norm(s.d.x, s.d.y, s.d.e);
+        s.h = xs_norm(s.h.x, s.h.y, s.h.e);
+    }
+    // a scaled value as a lazily normalised record (a zero mantissa takes
+    // the zero exponent, so that it never sets the exponent of a sum)
+    __device__ static xt tz(const xs &v) {
+        return (v.x == 0.f && v.y == 0.f) ? xt{0.f, 0.f, XT_ZERO_E} : as_t(v);
+    }
+    // z as a binary64 value
+    __device__ static cplx zd(const State &s) {
+        const int32_t e = max(s.z.e, -2000);
+        return mk(ldexp((double)s.z.x, e), ldexp((double)s.z.y, e));
+    }
+    // a new z from a binary64 value (rebase), its error carried over
+    __device__ static void set_z(State &s, double re, double im) {
+        const int32_t e0 = s.z.e;
+        s.z = xs_from(re, im, 0);
+        if (s.z.e == XS_ZERO_E) s.z.e = s.c.e;
+        const float f = KE * l1(s.z.x, s.z.y);
+        s.e2 = (float)ldexp((double)s.e2, max(min(2 * (e0 - s.z.e), 1000), -1100)) + f * f;
+        sync_c(s);
+    }
+    // the error bound of z, absolute
+    __device__ static double err_abs(const State &s) {
+        return ERR_SIGMAS * ldexp((double)__builtin_sqrtf(s.e2), max(s.z.e, -2000));
+    }
+
+    __device__ static void init(State &s, const KArgs &a, double px, double py) {
+        const double x1 = a.lin[0] * px + a.lin[1] * py;
+        const double y1 = a.lin[2] * px + a.lin[3] * py;
+        s.c = xs_from(a.dx_m * x1, a.dx_m * y1, a.dx_e) + xs_from(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
+        s.z = xs{0.f, 0.f, s.c.e == XS_ZERO_E ? 0 : s.c.e};
+        s.d = xs_zero(); s.h = xs_zero();
+        s.e2 = 0.f;
+        sync_c(s);
+        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
+    }
+
+    // one plain step on mantissas: the orbit point Zf (absolute, binary32),
+    // the path values Zd (dz/dc) and Zh (dz/dz) as normalised records
+    __device__ static void fast_step(State &s, const KArgs &a, float Zfx, float Zfy, const xs &Zd, const xs &Zh) {
+        const float ux = ldexpf(s.z.x, s.z.e), uy = ldexpf(s.z.y, s.z.e);     // z, absolute
+        const float sx = Zfx + ux, sy = Zfy + uy;                               // Z + z
+        float fp1;                                                              // |f'(Z + z)|^2
+        if constexpr (KIND == KIND_M2) {
+            fp1 = 4.f * (sx * sx + sy * sy);                                  // |f'|^2
+            // 2 ((Z + z) d + Zd z): both terms on the larger exponent
+            if (a.calc_dc) {
+                const int32_t E = max(s.d.e, Zd.e + s.z.e);
+                const float k1 = ldexpf(1.f, max(s.d.e - E, -200)), k2 = ldexpf(1.f, max(Zd.e + s.z.e - E, -200));
+                const float ax = k1 * s.d.x, ay = k1 * s.d.y, bx = k2 * Zd.x, by = k2 * Zd.y;
+                s.d.x = ax * sx - ay * sy + (bx * s.z.x - by * s.z.y);
+                s.d.y = ax * sy + ay * sx + (bx * s.z.y + by * s.z.x);
+                s.d.e = E + 1;
+            }
+            if (a.calc_dz) {
+                const int32_t E = max(s.h.e, Zh.e + s.z.e);
+                const float k1 = ldexpf(1.f, max(s.h.e - E, -200)), k2 = ldexpf(1.f, max(Zh.e + s.z.e - E, -200));
+                const float ax = k1 * s.h.x, ay = k1 * s.h.y, bx = k2 * Zh.x, by = k2 * Zh.y;
+                s.h.x = ax * sx - ay * sy + (bx * s.z.x - by * s.z.y);
+                s.h.y = ax * sy + ay * sx + (bx * s.z.y + by * s.z.x);
+                s.h.e = E + 1;
+            }
+            // z (z + 2 Z) + c
+            const float tx = 2.f * Zfx + ux, ty = 2.f * Zfy + uy;
+            const float nx = s.z.x * tx - s.z.y * ty + s.cx;
+            const float ny = s.z.x * ty + s.z.y * tx + s.cy;
+            const float fr = KE * (l1(s.z.x, s.z.y) * l1(tx, ty) + l1(s.cx, s.cy));
+            s.e2 = fp1 * s.e2 + fr * fr;
+            s.z.x = nx; s.z.y = ny;
+        } else {
+            // (Z + z)^d - Z^d + c by the binomial Horner scheme of p_zn, the
+            // running term in z's scale (its absolute value one ldexp away)
+            float mx = ux + (float)cbinom(DEG, 1) * Zfx, my = uy + (float)cbinom(DEG, 1) * Zfy;
+            float tx = s.z.x * mx - s.z.y * my, ty = s.z.x * my + s.z.y * mx;
+            float px = Zfx, py = Zfy;                      // Z^(q-1)
+#pragma unroll
+            for (int q = 2; q < DEG; q++) {
+                const float qx = px * Zfx - py * Zfy, qy = px * Zfy + py * Zfx;
+                px = qx; py = qy;
+                mx = ldexpf(tx, s.z.e) + (float)cbinom(DEG, q) * px;
+                my = ldexpf(ty, s.z.e) + (float)cbinom(DEG, q) * py;
+                tx = s.z.x * mx - s.z.y * my; ty = s.z.x * my + s.z.y * mx;
+            }
+            // |f'(Z + z)| = d |Z + z|^(d-1); the step's terms are bounded over
+            // |z| by d (|Z| + |z|)^(d-1) ((Z + z)^d - Z^d, mean value theorem)
+            float g = 1.f, gt = 1.f;
+            const float ls = l2(sx, sy), lt = l1(Zfx, Zfy) + l1(ux, uy);
+#pragma unroll
+            for (int q = 1; q < DEG; q++) { g *= ls; gt *= lt; }
+            fp1 = (float)DEG * g;
+            fp1 *= fp1;
+            if (a.calc_dc) s.d = dstep(s, Zd, s.d, ux, uy, Zfx, Zfy);
+            if (a.calc_dz) s.h = dstep(s, Zh, s.h, ux, uy, Zfx, Zfy);
+            const float fr = KE * ((float)(DEG * DEG) * gt * l1(s.z.x, s.z.y) + l1(s.cx, s.cy));
+            s.e2 = fp1 * s.e2 + fr * fr;
+            s.z.x = tx + s.cx; s.z.y = ty + s.cy;
+        }
+    }
+
+    // derivative delta of z^d + c (p_deriv of models.hpp) on the scaled
+    // mantissas: the result's exponent is T = max(e_d, e_Zd + e_z)
+    __device__ static xs dstep(const State &s, const xs &Zd, const xs &d, float ux, float uy,
+                               float Zfx, float Zfy) {
+        const int32_t T = max(d.e, Zd.e + s.z.e);
+        const float kd = ldexpf(1.f, max(d.e - T, -200)), kz = ldexpf(1.f, max(Zd.e + s.z.e - T, -200));
+        const float dx = kd * d.x, dy = kd * d.y;                    // d in scale T
+        const float zdx = kz * Zd.x, zdy = kz * Zd.y;
+        const float gx = zdx * s.z.x - zdy * s.z.y, gy = zdx * s.z.y + zdy * s.z.x;   // Zd z in scale T
+        const float C1 = (float)cbinom(DEG, 1);
+        float mx = ux + C1 * Zfx, my = uy + C1 * Zfy;                // z + C1 Z
+        float tmx = ux * mx - uy * my, tmy = ux * my + uy * mx;      // tmp = z mul, absolute
+        // dt = d mul + z d + C1 Zd z
+        float ex = dx * mx - dy * my + (ux * dx - uy * dy) + C1 * gx;
+        float ey = dx * my + dy * mx + (ux * dy + uy * dx) + C1 * gy;
+        float px = Zfx, py = Zfy;                                    // Z^(q-1)
+#pragma unroll
+        for (int q = 2; q < DEG; q++) {
+            // z dZpk = q Z^(q-1) Zd z
+            const float hx = (float)q * (px * gx - py * gy), hy = (float)q * (px * gy + py * gx);
+            const float qx = px * Zfx - py * Zfy, qy = px * Zfy + py * Zfx;
+            px = qx; py = qy;
+            const float Cq = (float)cbinom(DEG, q);
+            mx = tmx + Cq * px; my = tmy + Cq * py;
+            const float nx = dx * mx - dy * my + (ux * ex - uy * ey) + Cq * hx;
+            const float ny = dx * my + dy * mx + (ux * ey + uy * ex) + Cq * hy;
+            ex = nx; ey = ny;
+            tmx = ux * mx - uy * my; tmy = ux * my + uy * mx;
+        }
+        return xs{ex, ey, T};
+    }
+
+    __device__ static int step(State &s, const KArgs &a, Counters &k) {
+        const Tables &t = a.t;
+        const xs *Zx = (const xs *)a.tx.Zx, *DC = (const xs *)a.tx.DCx, *DZ = (const xs *)a.tx.DZx;
+        if ((s.w & 7) == 0) {
+            renorm(s);
+            while (t.bla_on && (s.w & 7) == 0) {
+                const double az = ldexp((double)hypotf(s.z.x, s.z.y), max(s.z.e, -2000));
+                const double eb = err_abs(s) + 0x1p-21 * az;
+                int32_t idx;
+                bool unsure = false;
+                const int32_t st = bla_find_m(t, az, eb, s.w, idx, unsure);
+                k.tries++;
+                if (unsure) return REASON_UNSURE;
+                if (st == 0) break;
+                // z <- A z + B c, d <- A d, h <- A h (A, B normalised records)
+                const xs A = ((const xs *)a.tx.Ax)[idx], B = ((const xs *)a.tx.Bx)[idx];
+                const int32_t E = max(A.e, B.e);
+                const float ka = ldexpf(1.f, max(A.e - E, -200)), kb = ldexpf(1.f, max(B.e - E, -200));
+                const float Ax = ka * A.x, Ay = ka * A.y, Bx = kb * B.x, By = kb * B.y;
+                const float nx = Ax * s.z.x - Ay * s.z.y + (Bx * s.cx - By * s.cy);
+                const float ny = Ax * s.z.y + Ay * s.z.x + (Bx * s.cy + By * s.cx);
+                const float fr = KE * (l1(Ax, Ay) * l1(s.z.x, s.z.y) + l1(Bx, By) * l1(s.cx, s.cy));
+                s.e2 = (Ax * Ax + Ay * Ay) * s.e2 + fr * fr;
+                s.z.x = nx; s.z.y = ny; s.z.e += E;
+                if (a.calc_dc) { float x, y; cmul(A.x, A.y, s.d.x, s.d.y, x, y); s.d = xs{x, y, s.d.e + A.e}; }
+                if (a.calc_dz) { float x, y; cmul(A.x, A.y, s.h.x, s.h.y, x, y); s.h = xs{x, y, s.h.e + A.e}; }
+                s.n += st;
+                s.w += st;                      // w + st < first_invalid <= order
+                renorm(s);
+                k.bla++; k.bla_it += st;
+            }
+        }
+        s.n += 1; k.plain++;
+        const xs Zr = Zx[s.w];
+        const xs Zd = (a.calc_dc && !s.dyn) ? DC[s.w] : xs_zero();
+        const xs Zh = a.calc_dz ? (s.nul ? DZ[0] : DZ[s.w]) : xs_zero();
+        constexpr int32_t LOW = (KIND == KIND_M2) ? -60 : -100 / (DEG - 1);
+        if (Zr.e >= LOW) {
+            fast_step(s, a, ldexpf(Zr.x, Zr.e), ldexpf(Zr.y, Zr.e), Zd, Zh);
+            if (l1(s.z.x, s.z.y) > RENORM_AT) renorm_z(s);
+            if (l1(s.d.x, s.d.y) > RENORM_AT) s.d = xs_norm(s.d.x, s.d.y, s.d.e);
+            if (l1(s.h.x, s.h.y) > RENORM_AT) s.h = xs_norm(s.h.x, s.h.y, s.h.e);
+        } else {
+            // the orbit point is below binary32's comfortable range: one step
+            // on lazily normalised records (products add exponents, sums align)
+            const xt z0 = tz(s.z), Zt = as_t(Zr);
+            const int32_t e0 = s.z.e;
+            if (a.calc_dc) s.d = of_t(p_deriv<KIND, DEG>(z0, tz(s.d), Zt, as_t(Zd)));
+            if (a.calc_dz) s.h = of_t(p_deriv<KIND, DEG>(z0, tz(s.h), Zt, as_t(Zh)));
+            const xs nz = of_t(p_zn<KIND, DEG>(z0, Zt, as_t(s.c)));
+            // |f'(Z + z)| <= d (|Z| + |z|)^(d-1), the step's terms <= |z| (|z| + d |Z|)^(d-1):
+            // magnitudes in units of 2^E, E the larger of |z|'s and |Z|'s exponents
+            const float mz = l2(s.z.x, s.z.y), mZ = l2(Zr.x, Zr.y);
+            const int32_t ez = mz == 0.f ? XS_ZERO_E : e0 + fexp(mz);
+            const int32_t eZ = mZ == 0.f ? XS_ZERO_E : Zr.e + fexp(mZ);
+            const int32_t E = max(max(ez, eZ), -(1 << 24));
+            const double lz = ldexp((double)mz, max(e0 - E, -1100)), lZ = ldexp((double)mZ, max(Zr.e - E, -1100));
+            double g = 1., gt = 1.;
+            for (int q = 1; q < DEG; q++) { g *= lz + lZ; gt *= lz + DEG * lZ; }
+            s.z = (nz.e == XS_ZERO_E) ? xs{0.f, 0.f, e0} : nz;
+            sync_c(s);
+            // in units of z's new scale
+            const double prop = ldexp(DEG * DEG * g * g * (double)s.e2,
+                                      max(min(2 * ((DEG - 1) * E + e0 - s.z.e), 1000), -1100));
+            const double fresh = ldexp(KE * lz * gt, max(min(DEG * E - s.z.e, 500), -1100));
+            const float fr = KE * (l1(s.z.x, s.z.y) + l1(s.cx, s.cy));
+            s.e2 = (float)(prop + fresh * fresh) + fr * fr;
+        }
+        s.w += 1;
+        if (s.w >= t.order) s.w = 0;
+        if (s.n >= a.max_iter) return 0;
+        if (a.calc_dz) {
+            const xs q = s.h + (s.nul ? DZ[0] : (s.n == t.order ? DZ[t.L] : DZ[s.w]));
+            const double q2 = ldexp(xs_abs2_m(q), 2 * max(q.e, -2000));
+            if (fabs(q2 - a.eps2) <= INTERIOR_MARGIN * a.eps2) return REASON_UNSURE;
+            if (q2 < a.eps2) return 2;
+        }
+        // escape and rebase tests in binary64, on the binary64 orbit
+        const cplx z64 = zd(s);
+        const cplx Zn1 = t.Zn[s.w];
+        const cplx ZZ = z64 + Zn1;
+        const double eabs = err_abs(s);
+        const double r2 = ZZ.re * ZZ.re + ZZ.im * ZZ.im;
+        // an error as large as a quarter of the escape radius leaves no decision certain
+        if (eabs * eabs > 0.0625 * a.M2) return REASON_UNSURE;
+        if (r2 > 0.25 * a.M2) {
+            // |ZZ| within its error bound of M: the binary64 loop may escape one step apart
+            const double m = sqrt(a.M2);
+            if (fabs(sqrt(r2) - m) <= eabs + 0x1p-50 * m) return REASON_UNSURE;
+            if (r2 > a.M2) return 1;
+        }
+        if (s.w >= t.div_iter - 1) {
+            set_z(s, ZZ.re, ZZ.im);
+            if (a.calc_dc) s.d = s.d + DC[s.w];
+            if (a.calc_dz) {
+                if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
+                s.nul = true;
+            }
+            s.w = 0;
+            k.reb++;
+            return -1;
+        }
+        // dynamic rebase: |ZZ| <= |z| component-wise; uncertain when a
+        // component lies within twice the error bound of its threshold
+        // (a zero orbit point leaves ZZ = z in both loops: certain)
+        const double dre = fabs(ZZ.re) - fabs(z64.re), dim = fabs(ZZ.im) - fabs(z64.im);
+        const double mg = 2. * eabs;
+        const bool zero_ref = (Zn1.re == 0. && Zn1.im == 0.);
+        if (!zero_ref && dre <= mg && dim <= mg && !(dre < -mg && dim < -mg)) return REASON_UNSURE;
+        s.dyn = (dre <= 0.) && (dim <= 0.);
+        if (s.dyn) {
+            // the binary64 loop confirms with the extended-range |ZZ|^2 <= |z|^2,
+            // which can decide otherwise only where z or the orbit point
+            // underflows binary64: left to that loop
+            if (fabs(z64.re) + fabs(z64.im) < 1e-290 || (zero_ref && Zx[s.w].e != XS_ZERO_E))
+                return REASON_UNSURE;
+            const xs ZZx = s.z + Zx[s.w];
+            if (xs_abs2_le(ZZx, s.z)) {
+                set_z(s, ZZ.re, ZZ.im);
+                if (a.calc_dc) s.d = s.d + DC[s.w];
+                if (a.calc_dz) {
+                    if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
+                    s.nul = true;
+                }
+                s.w = 0;
+                k.reb++;
+            }
+        }
+        return -1;
+    }
+
+    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
+        a.stop_reason[i] = (int8_t)reason;
+        if (reason == REASON_UNSURE) return;        // the binary64 pass writes this pixel
+        const int64_t n = a.ld;
+        const cplx z = zd(s) + a.t.Zn[s.w];
+        a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
+        int f = 1;
+        if (a.calc_dz) {
+            const cplx h = xs_to_d(xs_norm(s.h.x, s.h.y, s.h.e));
+            a.Z[2 * (n + i)] = h.re; a.Z[2 * (n + i) + 1] = h.im;
+            f = 2;
+        }
+        if (a.calc_dc) {
+            const cplx d = xs_to_d(xs_norm(s.d.x, s.d.y, s.d.e) + ((const xs *)a.tx.DCx)[s.w]);
+            a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
+        }
+        a.U[i] = s.w; a.stop_iter[i] = s.n;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // burning-ship family: real quantities as a binary32 mantissa in [1, 2)
 // (or 0) and an int32 exponent; the step and Jacobian formulas are the
 // binary64 kernels' own templates (models.hpp bs_zn / bs_hess,
@@ -574,7 +938,7 @@ static tile_fn select_bsx_lds(const LaunchCfg &c) {
 }
 
 static tile_fn select_f32(const LaunchCfg &c) {
-    if (c.kind == KIND_M2) return fn_of<HoloF32<KIND_M2, 2>>();
+    if (c.kind == KIND_M2) return fn_of<HoloS32<KIND_M2, 2>>();
     if (c.kind == KIND_BS) {
         switch (c.flavor) {
             case 1: return fn_of<BsF32<1>>();
@@ -587,7 +951,7 @@ static tile_fn select_f32(const LaunchCfg &c) {
     }
     if (c.kind == KIND_MN) {
         switch (c.degree) {
-#define FSG_MN(D) case D: return fn_of<HoloF32<KIND_MN, D>>();
+#define FSG_MN(D) case D: return fn_of<HoloS32<KIND_MN, D>>();
             FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
 #undef FSG_MN
             default: return nullptr;
@@ -662,10 +1026,10 @@ template <class Model> static tile_fn fn_lds_x(bool full) {
 }
 
 static tile_fn select_f32_lds(const LaunchCfg &c, bool full) {
-    if (c.kind == KIND_M2) return fn_lds_x<HoloF32<KIND_M2, 2>>(full);
+    if (c.kind == KIND_M2) return fn_lds_x<HoloS32<KIND_M2, 2>>(full);
     if (c.kind == KIND_MN) {
         switch (c.degree) {
-#define FSG_MN(D) case D: return fn_lds_x<HoloF32<KIND_MN, D>>(full);
+#define FSG_MN(D) case D: return fn_lds_x<HoloS32<KIND_MN, D>>(full);
             FSG_MN(2) FSG_MN(3) FSG_MN(4) FSG_MN(5) FSG_MN(6) FSG_MN(7) FSG_MN(8)
 #undef FSG_MN
             default: return nullptr;
@@ -684,12 +1048,9 @@ hipError_t launch_tile_f32(const LaunchCfg &cfg, const KArgs &a, hipStream_t str
         al.ldsx = bs ? plan_lds_bsx(a.t, max_lds_bytes()) : plan_lds_x(a.t, a.tx, max_lds_bytes());
         tile_fn fl = al.ldsx.bytes <= 0 ? nullptr
                      : bs ? select_bsx_lds(cfg) : select_f32_lds(cfg, al.ldsx.full != 0);
-        int per_cu = 0, dev = 0, ncu = 0;
-        if (fl && hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fl), 1024,
-                                                         (size_t)al.ldsx.bytes) == hipSuccess && per_cu > 0) {
-            hipLaunchKernelGGL(fl, dim3(launch_blocks(ncu * per_cu, al, 1024)), dim3(1024),
+        const int fill = fl ? fill_blocks(reinterpret_cast<const void *>(fl), 1024, al.ldsx.bytes) : -1;
+        if (fill > 0) {
+            hipLaunchKernelGGL(fl, dim3(launch_blocks(fill, al, 1024)), dim3(1024),
                                (size_t)al.ldsx.bytes, stream, al);
             return hipGetLastError();
         }

@@ -1,7 +1,14 @@
 """The binary32-mantissa + int32-exponent form against the binary64 drop-in
 on the same frames (the drop-in is bit-exact with the oracle, see
 test_gpu_parity.py): escape counts, stop reasons, continuous iteration and
-DEM, with the errors printed and bounded."""
+DEM, with the errors printed and bounded.
+
+Holomorphic models: every decision the binary32 loop is not certain of
+sends the pixel to a binary64 pass, so the escape counts are held to the
+1e-4 target everywhere (0 measured on every frame below).  Burning ship:
+no such pass; each bound is the measured mismatch with a margin of about
+1.5x, so that a regression fails, and the 1e-4 target is an expected
+failure on the deep frame."""
 import numpy as np
 import pytest
 
@@ -29,7 +36,8 @@ def test_fp32x_frames(name):
     args, c_pix = frames.frame_inputs(f)
     rep = compare(args, c_pix)
     assert rep["unfinished"] == 0 and rep["non_finite_zn"] == 0
-    assert rep["escape_count_mismatch_frac"] < 0.05
+    assert rep["escape_count_mismatch_frac"] <= 1e-4
+    assert rep["stop_reason_mismatch"] == 0
 
 
 @pytest.mark.parametrize("d", [3, 4, 6])
@@ -37,7 +45,7 @@ def test_fp32x_mn(d):
     f = frames.mn_frame(d, nx=128)
     args, c_pix = frames.frame_inputs(f)
     rep = compare(args, c_pix, d=d)
-    assert rep["unfinished"] == 0 and rep["escape_count_mismatch_frac"] < 0.05
+    assert rep["unfinished"] == 0 and rep["escape_count_mismatch_frac"] <= 1e-4
 
 
 @pytest.mark.parametrize("name", ["C1", "C2"])
@@ -51,10 +59,15 @@ def test_fp32x_workloads(name):
     assert rep["escape_count_mismatch_frac"] <= 1e-4
 
 
-@pytest.mark.parametrize("name", ["C3_d4", "C3_bs"])
+# measured escape-count mismatch x ~1.5 (burning ship: no binary64 pass)
+DEEP_BOUND = {"C3_d3": 1e-4, "C3_d4": 1e-4, "C3_d6": 1e-4, "C3_bs": 4e-4}
+
+
+@pytest.mark.parametrize("name", ["C3_d3", "C3_d4", "C3_d6", "C3_bs"])
 def test_fp32x_deep_workloads(name):
     """The 1e-800 frames (reduced to 480x270 pixels): BLA coefficients and
-    derivatives far outside the binary32 range."""
+    derivatives far outside the binary32 range.  Measured mismatch: 0 (z^d
+    + c), 2.5e-4 (burning ship)."""
     try:
         configs.center_for(name, compute=False)
     except KeyError:
@@ -66,37 +79,58 @@ def test_fp32x_deep_workloads(name):
     print(f"case fp32x {name} npts={c_pix.size}", flush=True)
     rep = compare(args, c_pix, d=w.degree if w.model == "MN" else 2)
     assert rep["unfinished"] == 0 and rep["non_finite_zn"] == 0
-    assert rep["escape_count_mismatch_frac"] < 0.01
+    assert rep["escape_count_mismatch_frac"] <= DEEP_BOUND[name]
     assert rep["stop_reason_mismatch"] == 0
 
 
-@pytest.mark.parametrize("name,flavor", [("BS_ex14", "Burning ship"), ("PBS_ex18", "Perpendicular burning ship")])
+@pytest.mark.xfail(strict=True, reason="the burning-ship binary32 form has no binary64 pass over "
+                   "its uncertain pixels: 2.5e-4 of the escape counts move at 1e-800")
+def test_fp32x_burning_ship_deep_target():
+    w = configs.WORKLOADS["C3_bs"]
+    try:
+        configs.center_for("C3_bs", compute=False)
+    except KeyError:
+        pytest.skip("no cached seeded centre for C3_bs")
+    f = configs.build(w, nx=480, ny=270)
+    args = f._calc_data["bench"]["cycle_indep_args"]
+    c_pix = np.ascontiguousarray(np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))))
+    rep = compare(args, c_pix)
+    assert rep["escape_count_mismatch_frac"] <= 1e-4
+
+
+@pytest.mark.parametrize("name,flavor,bound", [("BS_ex14", "Burning ship", 0.20),
+                                               ("PBS_ex18", "Perpendicular burning ship", 0.03)])
 @pytest.mark.parametrize("xr", [0, 1])
-def test_fp32x_burning_ship_deep(name, flavor, xr):
+def test_fp32x_burning_ship_deep(name, flavor, bound, xr):
     """The burning-ship family in the binary32 form (the binary64 kernels'
     step and Jacobian templates on a binary32 mantissa with an int32
     exponent) on the reference examples' deep frames.  These frames are
     chaotic at binary64 precision already: switching BLA off moves 9 % (BS)
     and 0.2 % (PBS) of the escape counts of the drop-in, and binary32 moves
-    16 % and 2 %; the continuous iteration of the escaped pixels agrees to
-    ~1e-8 (median)."""
+    16.5 % and 2.2 % (the bounds: x 1.2 and x 1.4); the continuous
+    iteration of the escaped pixels agrees to ~1e-8 (median)."""
     f = frames.make(name, nx=96, model_kw={"flavor": flavor}, max_iter=6000)
     args, c_pix = frames.frame_inputs(f)
     args = args.with_fields(xr_mode=xr)
     print(f"case fp32x {name} {flavor} xr={xr}", flush=True)
     rep = compare(args, c_pix)
     assert rep["unfinished"] == 0 and rep["non_finite_zn"] == 0
-    assert rep["escape_count_mismatch_frac"] < 0.25
-    assert rep["stop_reason_mismatch"] <= 0.01 * rep["pixels"]
+    assert rep["escape_count_mismatch_frac"] < bound
+    assert rep["stop_reason_mismatch"] <= 0.002 * rep["pixels"]
     assert rep["nu_rel_err_median"] < 1e-5 and rep["nu_rel_err_p99"] < 1e-3
 
 
-@pytest.mark.parametrize("flavor", ["Burning ship", "Perpendicular burning ship", "Shark fin", "Celtic", "Buffalo"])
+# measured: 2.30 %, 0.58 %, 0.47 %, 2.03 %, 1.78 % of the escape counts
+FLAVOUR_BOUND = {"Burning ship": 0.035, "Perpendicular burning ship": 0.009, "Shark fin": 0.007,
+                 "Celtic": 0.03, "Buffalo": 0.027}
+
+
+@pytest.mark.parametrize("flavor", list(FLAVOUR_BOUND))
 def test_fp32x_burning_ship_flavours(flavor):
     """Every flavour in the binary32 form on a whole-set view (perturbation
     around the critical orbit, as the reference does above 1e-5)."""
     import fractalshades_amd as fsa
-    f = fsa.Perturbation_burning_ship("/tmp/fsa_test", flavor=flavor)
+    f = fsa.Perturbation_burning_ship(None, flavor=flavor)
     f.zoom(precision=20, x="-0.5", y="-0.5", dx="3.0", nx=96, xy_ratio=1.0, theta_deg=0.)
     f.calc_std_div(calc_name="t", subset=None, max_iter=2000, M_divergence=1.e3, BLA_eps=1.e-6,
                    calc_hessian=True)
@@ -104,8 +138,8 @@ def test_fp32x_burning_ship_flavours(flavor):
     print(f"case fp32x flavour {flavor}", flush=True)
     rep = compare(args, c_pix)
     assert rep["unfinished"] == 0 and rep["non_finite_zn"] == 0
-    assert rep["escape_count_mismatch_frac"] < 0.05
-    assert rep["stop_reason_mismatch"] <= 0.01 * rep["pixels"]
+    assert rep["escape_count_mismatch_frac"] < FLAVOUR_BOUND[flavor]
+    assert rep["stop_reason_mismatch"] <= 0.003 * rep["pixels"]
     assert rep["nu_rel_err_median"] < 1e-5 and rep["nu_rel_err_p99"] < 1e-3
 
 
