@@ -219,75 +219,6 @@ struct HoloXr {
         s.orb.reset(a.backshift);
     }
 
-    // z^2 + c step in plain binary64 arithmetic on the mantissas, exactly
-    // as the extended-range step rounds.  Every extended-range operation
-    // rounds the exact result of its operands once, wherever the split
-    // between mantissa and exponent lies, unless an operand aligned for a
-    // sum turns subnormal while it still matters.  With the orbit point
-    // normal and far above the subnormal range (|Z| >= 2^-800) and no
-    // massive cancellation in Z + z or z + 2 Z (each at least 2^-60 of its
-    // larger operand), the step's terms are either within 2^-60 of each
-    // other's scale or negligible (below half an ulp of the other), so:
-    //   z + 2 Z and Z + z are binary64 sums of z's absolute value u (one
-    //   ldexp; an underflow of u only drops a negligible term) and Z;
-    //   z (z + 2 Z) + c is a product and a sum at z's own exponent;
-    //   2 ((Z + z) d + Zd z) takes its two terms on the larger exponent.
-    // The results keep their mantissas unnormalised (d and z renormalised
-    // when they leave [2^-60, 2^60], every value at the next BLA point).
-    // Returns false, with nothing changed, where these conditions fail:
-    // the extended-range step then runs as written.
-    __device__ static bool fast_m2(State &s, const KArgs &a) {
-        const Tables &t = a.t;
-        if (t.xr_pos && t.xr_pos[s.w] >= 0) return false;           // orbit point in the register
-        if (s.z.e == XL_ZERO_E) return false;
-        const cplx Zb = t.Zn[s.w];
-        const cplx u = mk(ldexp(s.z.re, s.z.e), ldexp(s.z.im, s.z.e));
-        const cplx Zu = Zb + u;
-        const cplx t2 = u + mk(2. * Zb.re, 2. * Zb.im);
-        const double mZ = fmax(fabs(Zb.re), fabs(Zb.im)), mu = fmax(fabs(u.re), fabs(u.im));
-        const double big = 0x1p-60 * fmax(mZ, mu);
-        if (!(mZ >= 0x1p-800 && fmax(fabs(Zu.re), fabs(Zu.im)) >= big &&
-              fmax(fabs(t2.re), fabs(t2.im)) >= big))
-            return false;
-        if (a.calc_dc) s.d = dstep(s.d, Zu, s.dyn ? xz_zero() : DCz(t, s.w), s.z);
-        if (a.calc_dz) s.h = dstep(s.h, Zu, s.nul ? DZz(t, 0) : DZz(t, s.w), s.z);
-        // z (z + 2 Z) + c, c on z's exponent (a c below 2^-1022 there is negligible)
-        const cplx p = mk(s.z.re, s.z.im) * t2;
-        const int32_t sc = max(s.c.e - s.z.e, -2200);
-        s.z.re = p.re + ldexp(s.c.re, sc);
-        s.z.im = p.im + ldexp(s.c.im, sc);
-        double mx = fmax(fabs(s.z.re), fabs(s.z.im));
-        if (!(mx >= 0x1p-60 && mx <= 0x1p60)) {
-            s.z = xl_n(s.z.re, s.z.im, s.z.e);
-            mx = fmax(fabs(s.z.re), fabs(s.z.im));
-        }
-        // the binary64 value: to_std's two products by 2^e (an exact ldexp of
-        // the unnormalised mantissa) unless the value is near binary64's
-        // bottom, its components are more than 2^1020 apart or a component
-        // comes out zero, where to_std runs the reference's procedure on the
-        // normalised value
-        const double mn = fmin(fabs(s.z.re), fabs(s.z.im));
-        s.zs = mk(ldexp(s.z.re, s.z.e), ldexp(s.z.im, s.z.e));
-        if (s.z.e + dexp(mx) < -1000 || mn < 0x1p-1020 * mx || s.zs.re == 0. || s.zs.im == 0.)
-            s.zs = to_std(xl_n(s.z.re, s.z.im, s.z.e));
-        return true;
-    }
-    // 2 ((Z + z) d + Zd z) on binary64 mantissas, both terms on the larger exponent
-    __device__ static xl dstep(const xl &d, const cplx Zu, const xz Zd, const xl &z) {
-        const cplx A = Zu * mk(d.re, d.im);
-        const cplx B = mk(Zd.re, Zd.im) * mk(z.re, z.im);
-        const int32_t eb = Zd.e + z.e;
-        const int32_t E = max(d.e, eb);
-        const int32_t sa = max(d.e - E, -2200), sb = max(eb - E, -2200);
-        xl r;
-        r.re = ldexp(A.re, sa) + ldexp(B.re, sb);
-        r.im = ldexp(A.im, sa) + ldexp(B.im, sb);
-        r.e = E + 1;                                      // the factor 2, exactly
-        const double m = fmax(fabs(r.re), fabs(r.im));
-        if (!(m >= 0x1p-60 && m <= 0x1p60)) return xl_n(r.re, r.im, r.e);
-        return r;
-    }
-
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
         while (t.bla_on && (s.w & 7) == 0) {
@@ -308,18 +239,12 @@ struct HoloXr {
             }
         }
         s.n += 1; k.plain++;
-        if constexpr (KIND == KIND_M2) {
-            if (fast_m2(s, a)) goto stepped;
-        }
-        {
-            const xz Zr = as_z(ZR(t, s.w));
-            const xz z0 = as_z(s.z);
-            if (a.calc_dc) s.d = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.d), Zr, s.dyn ? xz_zero() : DCz(t, s.w)));
-            if (a.calc_dz) s.h = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.h), Zr, s.nul ? DZz(t, 0) : DZz(t, s.w)));
-            s.z = xl_of(p_zn<KIND, DEG>(z0, Zr, as_z(s.c)));
-            s.zs = to_std(s.z);
-        }
-    stepped:
+        const xz Zr = as_z(ZR(t, s.w));
+        const xz z0 = as_z(s.z);
+        if (a.calc_dc) s.d = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.d), Zr, s.dyn ? xz_zero() : DCz(t, s.w)));
+        if (a.calc_dz) s.h = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.h), Zr, s.nul ? DZz(t, 0) : DZz(t, s.w)));
+        s.z = xl_of(p_zn<KIND, DEG>(z0, Zr, as_z(s.c)));
+        s.zs = to_std(s.z);
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;
