@@ -91,6 +91,8 @@ class PostDesc(ctypes.Structure):
         ("d", ctypes.c_double), ("a_d", ctypes.c_double), ("M", ctypes.c_double),
         ("floor_iter", ctypes.c_double), ("px_snap", ctypes.c_double),
         ("skew", ctypes.c_double * 4),
+        ("proj", ctypes.c_int32), ("mod", ctypes.c_int32), ("w_scale", ctypes.c_double),
+        ("mod_scale", ctypes.c_double),
     ]
 
 
@@ -185,7 +187,7 @@ def load(path=None):
             "fsg_server_submit": ([vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
                                   ctypes.c_int),
             "fsg_server_wait": ([vp, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)], ctypes.c_int),
-            "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp, vp],
+            "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp, vp, vp],
                                     ctypes.c_int),
             "fsg_server_submit_layers": ([vp, vp, vp, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp, vp,
                                           ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
@@ -201,7 +203,7 @@ def load(path=None):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.fsg_abi_version() != 4:
+        if lib.fsg_abi_version() != 5:
             raise FsgError("libfsgpu.so ABI version mismatch")
         if path is None:
             _lib = lib

@@ -49,8 +49,8 @@ size_t ctx_z_bytes(const fsg_ctx *c, int64_t n);
 int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, double *Z, int32_t *U,
                      int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks);
 size_t work_words();
-int post_launch(const fsg_post_desc *pp, const double *Z, int64_t n, int64_t ld, const int32_t *si,
-                const int8_t *sr, float *nu, float *dem, float *nrm, hipStream_t stream);
+int post_launch(const fsg_post_desc *pp, const double *Z, int64_t n, int64_t ld, const double *pix,
+                const int32_t *si, const int8_t *sr, float *nu, float *dem, float *nrm, hipStream_t stream);
 }  // namespace fsg
 
 using namespace fsg;
@@ -258,6 +258,7 @@ void fsg_server::loop() {
                     float *dem = r->dem_h ? reinterpret_cast<float *>(r->dev + r->lay.dem) : nullptr;
                     float *nrm = r->nrm_h ? reinterpret_cast<float *>(r->dev + r->lay.nrm) : nullptr;
                     const int prc = post_launch(&r->pp, reinterpret_cast<const double *>(r->dev + r->lay.z), n, n,
+                                                reinterpret_cast<const double *>(r->dev + r->lay.pix),
                                                 reinterpret_cast<const int32_t *>(r->dev + r->lay.si),
                                                 reinterpret_cast<const int8_t *>(r->dev + r->lay.sr), nu, dem, nrm,
                                                 copy);

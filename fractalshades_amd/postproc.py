@@ -66,6 +66,23 @@ def normal_map(zn, dzndc=None, jac=None, skew=None):
     return normal / np.abs(normal)
 
 
+def screen_derivatives(projection, c_pix, dzndc=None, jac=None):
+    """The derivatives as the layers use them (get_dzndc, postproc.py:184-206):
+    dz/dc times the projection's df at the pixel (apply_df, postproc.py:
+    972-978), or the burning-ship Jacobian times its dfBS matrix
+    (apply_dfBS, postproc.py:980-998: dX/dpix_a = dX/dA m00 + dX/dB m10,
+    dX/dpix_b = dX/dA m01 + dX/dB m11, the same for Y).  The identity for a
+    Cartesian frame."""
+    if projection is None or getattr(projection, "df_kind", 0) == 0:
+        return dzndc if jac is None else jac
+    if jac is None:
+        return projection.df(c_pix) * np.asarray(dzndc, dtype=np.complex128)
+    m00, m01, m10, m11 = projection.dfBS(c_pix)
+    dXdA, dXdB, dYdA, dYdB = (np.asarray(v, dtype=np.float64) for v in jac)
+    return (dXdA * m00 + dXdB * m10, dXdA * m01 + dXdB * m11,
+            dYdA * m00 + dYdB * m10, dYdA * m01 + dYdB * m11)
+
+
 def fields_of(Z, codes, holomorphic=True):
     """Split a raw Z array into named fields, following the codes."""
     names = codes[0]
@@ -123,13 +140,10 @@ def post_desc(f, calc_name, floor_iter=0., px_snap=None):
     """fsg_post_desc of a calculation: potential parameters of the model
     (potential_d, potential_a_d, potential_M, as Continuous_iter_pp reads
     them, postproc.py:320-347), the row of dz/dc (dX/da for the burning
-    ship) in its raw Z array, and the frame's skew for the normal map."""
+    ship) in its raw Z array, the frame's skew for the normal map and the
+    projection's map of the derivatives (screen_derivatives), as it stands
+    (whole map, or the current band of a map computed by bands)."""
     from ._native import PostDesc
-    from .projection import is_cartesian
-    if not is_cartesian(getattr(f, "projection", None)):
-        # the reference's layers rotate dz/dc with the projection's df
-        # (postproc.py:184-206), which the device layers do not
-        raise NotImplementedError("device layers of a non-Cartesian projection")
     state = f._calc_data[calc_name]["state"]
     names = list(state.codes[0])
     holo = bool(getattr(f, "holomorphic", True))
@@ -143,4 +157,29 @@ def post_desc(f, calc_name, floor_iter=0., px_snap=None):
     sk = np.ravel(skew) if skew is not None else (1., 0., 0., 1.)
     for i in range(4):
         pp.skew[i] = float(sk[i])
+    proj = getattr(f, "projection", None)
+    kind = int(getattr(proj, "df_kind", 0)) if proj is not None and not isinstance(proj, str) else 0
+    pp.proj = kind
+    if kind in (1, 2):
+        # the whole map's offsets are exp(h_moy + t): exp(t) = w exp(-h_moy)
+        pp.w_scale = float(np.exp(-proj.hmoy))
+    mod, mod_scale = derivative_modifier(proj)
+    pp.mod, pp.mod_scale = mod, mod_scale
     return pp
+
+
+def derivative_modifier(proj):
+    """The loop's dzndc_modifier (projection.py:205-217, 454-471) as a
+    function of the pixel's projected offset w, for the device layers:
+    (0, 0) none; (1, s): |w| s, the exponential map (w = exp(h_moy + t) over
+    the whole map or by bands of a map within binary64, where the modifier is
+    exp(Re t + h_moy - h_band); w = exp(h_moy - h_band + t) by bands of a
+    deep map); (2, seam): |w + 1e-6| seam, the Cartesian expmap seam."""
+    from .projection import Cartesian, Expmap
+    if isinstance(proj, Expmap):
+        if proj.use_step and not proj.deep:
+            return 1, float(np.exp(-proj.exp_step_hmoy))
+        return 1, 1.
+    if isinstance(proj, Cartesian) and proj.expmap_seam is not None:
+        return 2, float(proj.expmap_seam)
+    return 0, 0.

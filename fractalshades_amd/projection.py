@@ -54,6 +54,28 @@ class Projection:
     def min_local_scale(self):
         raise NotImplementedError
 
+    # The map the post-processing layers apply to dz/dc (the Jacobian for the
+    # burning ship) before the DEM and the normal (get_dzndc, postproc.py:
+    # 184-206): none for the identity.  DF_* codes as in fsg_post_desc.proj.
+    df_kind = 0
+
+    def df(self, pix):
+        """Complex factor of dz/dc at the pixels (None: no factor)."""
+        return None
+
+    def dfBS(self, pix):
+        """(m00, m01, m10, m11) applied to the burning-ship Jacobian (None: none)."""
+        return None
+
+
+# df kinds (fsg_post_desc.proj): the factor of the holomorphic dz/dc and the
+# matrix of the burning-ship Jacobian, t = pix_to_ht * pix
+DF_NONE = 0          # Cartesian
+DF_EXP = 1           # exp(t);            (-Im e^t, -Re e^t, Re e^t, -Im e^t)
+DF_EXP_REAL = 2      # exp(Re t);         (e^Re t, 0, 0, e^Re t)
+DF_ROT = 3           # exp(i Im t);       (cos Im t, -sin Im t, sin Im t, cos Im t)
+DF_ONE = 4           # 1;                 identity
+
 
 class Cartesian(Projection):
     """The identity (projection.py:212-266)."""
@@ -161,6 +183,47 @@ class Expmap(Projection):
         if self.deep and self.use_step:
             return mpmath.exp(self.exp_step_hmoy)
         return 1
+
+    @property
+    def df_kind(self):
+        """The post-processing map of the derivatives (projection.py:375-453):
+        by bands (use_step) the rotation exp(i Im t) only, the scale being
+        applied by dzndc_modifier in the loop; over the whole map exp(t), or
+        exp(Re t) without rotates_df.  (Over the whole map the reference's
+        make_dfBS_impl stores its matrix in `df` as well, projection.py:435,
+        which its holomorphic layers then cannot apply; here each model keeps
+        its own map as make_df_impl / make_dfBS_impl write it.)"""
+        if self.use_step:
+            return DF_ROT if self.rotates_df else DF_ONE
+        return DF_EXP if self.rotates_df else DF_EXP_REAL
+
+    def df(self, pix):
+        t = self.pix_to_ht * np.asarray(pix, dtype=np.complex128)
+        k = self.df_kind
+        if k == DF_EXP:
+            return np.exp(t)
+        if k == DF_EXP_REAL:
+            return np.exp(np.real(t)).astype(np.complex128)
+        if k == DF_ROT:
+            return np.exp(1j * np.imag(t))
+        return np.ones_like(t)
+
+    def dfBS(self, pix):
+        t = self.pix_to_ht * np.asarray(pix, dtype=np.complex128)
+        k = self.df_kind
+        if k == DF_EXP:
+            r = np.exp(np.real(t))
+            cr, sr = np.cos(np.imag(t)) * r, np.sin(np.imag(t)) * r
+            return -sr, -cr, cr, -sr
+        if k == DF_EXP_REAL:
+            r = np.exp(np.real(t))
+            z = np.zeros_like(r)
+            return r, z, z, r
+        if k == DF_ROT:
+            c, s = np.cos(np.imag(t)), np.sin(np.imag(t))
+            return c, -s, s, c
+        o, z = np.ones(t.shape), np.zeros(t.shape)
+        return o, z, z, o
 
     def dzndc_modifier(self, cpix):
         """exp(Re(pix_to_ht * cpix) + h_shift), h_shift = h_moy, or

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 4
+#define FSG_ABI_VERSION 5
 
 /* model kinds */
 #define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
@@ -252,16 +252,33 @@ typedef struct fsg_post_desc {
     double floor_iter;       /* Continuous_iter_pp floor_iter */
     double px_snap;          /* DEM_pp px_snap; <= 0 for none */
     double skew[4];          /* the frame's skew (s00, s01, s10, s11); all zero = none */
+    /* The projection's factors of the derivatives, applied before the DEM and
+     * the normal.  First the loop's dzndc_modifier (perturbation.py:1387-1388,
+     * 1773-1777; projection.py:205-217, 454-471), which the tile kernels leave
+     * out: mod 0 none; 1 |w| * mod_scale (exponential map); 2 |w + 1e-6| *
+     * mod_scale (the Cartesian expmap seam), w the pixel's projected offset.
+     * Then the layers' map (get_dzndc / apply_df / apply_dfBS, postproc.py:
+     * 184-206, 972-998; projection.py:375-453), with t = pix_to_ht * pix at the
+     * screen pixel: proj 0 none; 1 exp(t); 2 exp(Re t); 3 exp(i Im t); 4 one;
+     * the burning ship's Jacobian takes the matching 2x2 matrix.  The device
+     * forms them from w = exp(h + t): exp(t) = w * w_scale, exp(i Im t) = w / |w|. */
+    int32_t proj;
+    int32_t mod;
+    double w_scale;          /* exp(-h_moy) of the whole map (maps 1 and 2) */
+    double mod_scale;
 } fsg_post_desc;
 
 /* Enqueue the layers of n pixels (Z rows of stride ld >= n) on stream;
  * nu, dem or normal may be null to skip that layer.  normal: float32 [n][2],
  * the unit normal of the potential map (Attr_normal_pp kind "potential",
  * postproc.py:572-628: z_n / (dz_n/dc), for the burning ship J^T z_n; then
- * unskewed with the transposed skew, core.py:3147-3158, and normalised). */
+ * unskewed with the transposed skew, core.py:3147-3158, and normalised).
+ * d_c_pix: the tile's pixels as the tile kernel took them (complex128 [n]:
+ * the projected offsets), read when pp->proj or pp->mod is not 0 (else may be
+ * null). */
 int fsg_postproc_device(const fsg_post_desc *pp, const double *d_Z, int64_t n, int64_t ld,
-                        const int32_t *d_stop_iter, const int8_t *d_stop_reason, float *d_nu,
-                        float *d_dem, float *d_normal, void *stream);
+                        const double *d_c_pix, const int32_t *d_stop_iter, const int8_t *d_stop_reason,
+                        float *d_nu, float *d_dem, float *d_normal, void *stream);
 
 /*
  * ---- Serving loop ----------------------------------------------------------

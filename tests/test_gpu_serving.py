@@ -186,6 +186,71 @@ def test_server_layers_match_host_postproc(name, model_kw):
     t.close()
 
 
+@pytest.mark.parametrize("name", ["M2_E20", "BS_ex14"])
+@pytest.mark.parametrize("banded", [False, True])
+def test_server_layers_expmap(name, banded):
+    """The device layers of an exponential-map frame: the derivatives take
+    the loop's dzndc_modifier and the projection's df (over the whole map
+    exp(t); by bands exp(i Im t); the burning ship's 2x2 matrices), formed on
+    the device from each pixel's projected offset, equal the host formulas
+    (postproc.screen_derivatives on the screen pixels) to float32
+    resolution."""
+    from fractalshades_amd.postproc import continuous_iter, dem, normal_map, post_desc, screen_derivatives
+    from fractalshades_amd.serving import LayerTile
+    kw = {"max_iter": 6000} if name.startswith("BS") else {}
+    f = frames.expmap_frame(name, nx=256, hmin=0., hmax=8., **kw)
+    p = f.projection
+    a, _ = frames.frame_inputs(f)
+
+    def check(a, pix, pp):
+        w = np.ascontiguousarray(p.f(pix))
+        (Z, U, sr, si), _ = run_gpu(a, w)
+        esc = sr[0] == 1
+        assert esc.sum() > 50
+        holo = a.fields["kind"] != 2
+        zn = Z[0] if holo else Z[0] + 1j * Z[1]
+        mod = p.dzndc_modifier(pix)
+        if holo:
+            dz = screen_derivatives(p, pix[esc], dzndc=Z[-1][esc] * mod[esc])
+            dem_ref, nrm_ref = dem(zn[esc], dzndc=dz), normal_map(zn[esc], dzndc=dz, skew=f._skew)
+        else:
+            jac = screen_derivatives(p, pix[esc], jac=tuple(Z[k][esc] * mod[esc] for k in range(2, 6)))
+            dem_ref, nrm_ref = dem(zn[esc], jac=jac), normal_map(zn[esc], jac=jac, skew=f._skew)
+        nu_ref = continuous_iter(zn[esc], si[0][esc], d=f.potential_d, a_d=f.potential_a_d, M=1e3)
+        t = LayerTile(w.size, normal=True)
+        t.c_pix[:] = w
+        srv = TileServer(lanes=2, chunk_px=1024)
+        try:
+            info = srv.wait(srv.submit_layers(a.ctx, t, pp), 60000)
+        finally:
+            srv.close()
+        assert info["status"] == 0
+        np.testing.assert_array_equal(t.stop_iter, si)
+        np.testing.assert_allclose(t.nu[esc], nu_ref.astype(np.float32), rtol=2e-7, atol=0)
+        np.testing.assert_allclose(t.dem[esc], dem_ref.astype(np.float32), rtol=5e-7, atol=0)
+        np.testing.assert_allclose(t.normal[esc, 0], nrm_ref.real.astype(np.float32), rtol=0, atol=5e-7)
+        np.testing.assert_allclose(t.normal[esc, 1], nrm_ref.imag.astype(np.float32), rtol=0, atol=5e-7)
+        t.close()
+
+    if not banded:
+        pp = post_desc(f, "t")
+        assert pp.proj == 1 and pp.mod == 1
+        check(a, np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny))), pp)
+        return
+    from fractalshades_amd.perturbation import CycleIndepArgs
+    n_bands = 0
+    for fields, arrays, chunks in f.expmap_bands("t", exp_zoom_step=200):
+        pp = post_desc(f, "t")
+        assert pp.proj == 3 and pp.mod == 1
+        ab = CycleIndepArgs(fields, arrays, None, projection=p).shared_with(a.ctx)
+        pix = np.concatenate([np.ravel(f.chunk_pixel_pos(ch)) for ch in chunks])
+        print(f"case layers expmap band {n_bands} of {name}", flush=True)
+        check(ab, pix, pp)
+        ab.close()
+        n_bands += 1
+    assert n_bands >= 2
+
+
 def test_server_wait_from_several_threads():
     """Two threads waiting on one ticket both get the request's status and
     times; the ticket is released after them, so a later wait fails cleanly

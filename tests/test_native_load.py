@@ -23,7 +23,7 @@ def test_library_exports_all_declared():
     lib = native.load()
     for s in declared_symbols():
         assert hasattr(lib, s), s
-    assert lib.fsg_abi_version() == 4
+    assert lib.fsg_abi_version() == 5
     assert lib.fsg_work_bytes() >= 40
 
 
@@ -37,6 +37,9 @@ def test_desc_layout_matches_header():
     assert d.calc_orbit.offset == 212 and d.backshift.offset == 216
     assert d.deriv_mant.offset == 224 and d.deriv_exp.offset == 232
     assert native.ctypes.sizeof(d) == 240
+    p = native.PostDesc
+    assert p.skew.offset == 48 and p.proj.offset == 80 and p.mod.offset == 84
+    assert p.w_scale.offset == 88 and p.mod_scale.offset == 96 and native.ctypes.sizeof(p) == 104
 
 
 def test_argument_errors_without_gpu():

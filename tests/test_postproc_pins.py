@@ -124,3 +124,54 @@ def test_burning_ship_nu_dem_normal_vs_potential(name):
         assert de * dx <= de_lim * (1. + 1e-6)
         nrm = normal_map(np.array([zn]), jac=jac)[0]
         assert abs(nrm - grad_dir) < 1e-5, (i, nrm, grad_dir)
+
+
+def test_expmap_screen_derivative_vs_brute_force():
+    """Exponential map by bands (set_exp_zoom_step): the derivative the
+    layers use -- the loop's dz/dc (scaled by dx exp(h_band)), times the
+    band's dzndc_modifier exp(Re t + h_moy - h_band), times df = exp(i Im t)
+    (screen_derivatives), t = pix_to_ht pix -- is the derivative of z_n with
+    respect to the screen position divided by pix_to_ht: c = centre +
+    dx exp(h_moy + t) gives dz_n/dpix = dz_n/dc dx pix_to_ht exp(h_moy + t).
+    Checked on sampled pixels of every band against central differences of
+    the brute-force z_n (n fixed) in pix, within 1e-6 (the step moves z_n by
+    ~1e-5, where z_n is linear in pix to ~1e-8)."""
+    from fractalshades_amd.postproc import screen_derivatives
+    from fractalshades_amd import settings
+    f = frames.expmap_frame("M2_E20", nx=400, hmin=0., hmax=10., dx="1e-24")
+    p = f.projection
+    assert f.nx // settings.chunk_size >= 2
+    rng = np.random.default_rng(5)
+    checked = n_bands = 0
+    for fields, arrays, chunks in f.expmap_bands("t", exp_zoom_step=settings.chunk_size):
+        n_bands += 1
+        assert p.use_step and p.df_kind == 3
+        for ix, ixx, iy, iyy in chunks[:4]:
+            pix = np.ravel(f.chunk_pixel_pos((ix, ixx, iy, iyy)))
+            pix = pix[rng.choice(pix.size, min(8, pix.size), replace=False)]
+            off = np.ascontiguousarray(p.f(pix))
+            Zo, Uo, sro, sio = orc.OracleCtx(dict(fields, bla_on=0), arrays).cycles(off)
+            dz = Zo[-1] * p.dzndc_modifier(pix)
+            scr = screen_derivatives(p, pix, dzndc=dz)
+            for q in range(pix.size):
+                n = int(sio[0, q])
+                if sro[0, q] != 1:
+                    continue
+                with mpmath.workprec(f.prec + 300):
+                    # a step that moves z_n by ~1e-5 (|z_n| ~ 1e3): in the linear regime
+                    h = mpmath.mpf(1e-8) * 1e3 / max(abs(scr[q]), 1.)
+                    zs = []
+                    for sgn in (1, -1, 0):
+                        t = (mpmath.mpc(complex(pix[q])) + sgn * h) * mpmath.mpf(p.pix_to_ht)
+                        c = mpmath.mpc(f.x, f.y) + f.dx * mpmath.exp(mpmath.mpf(p.hmoy) + t)
+                        z = mpmath.mpc(0)
+                        for _ in range(n):
+                            z = z * z + c
+                        zs.append(z)
+                    fd = complex((zs[0] - zs[1]) / (2 * h) / mpmath.mpf(p.pix_to_ht))
+                # the binary64 loop tracks the exact orbit (else rounding decides)
+                if abs(complex(zs[2]) - (Zo[0, q])) > 1e-6 * abs(complex(zs[2])):
+                    continue
+                assert abs(scr[q] - fd) <= 1e-6 * abs(fd), (n, scr[q], fd)
+                checked += 1
+    assert n_bands >= 2 and checked >= 8
