@@ -79,7 +79,8 @@ typedef struct fsg_ref_desc {
     int32_t kc_exp;
     int32_t bla_on;          /* BLA_activated (mandelbrot_M2.py:521-524) */
     double bla_eps;          /* BLA_eps */
-    int64_t max_iter;
+    int64_t max_iter;        /* stop_iter is int32: max_iter + ref_len must stay below 2^31
+                                (FSG_E_INVALID otherwise) */
     double M_divergence;
     double epsilon_stationnary;
     int32_t calc_dzndc;      /* holomorphic calc_dzndc; burning ship calc_hessian */
