@@ -231,33 +231,54 @@ struct HoloXr {
                 cplx A = mk(Mb[0], Mb[1]), B = mk(Mb[2], Mb[3]);
                 s.n += st;
                 s.w += st;
-                s.z = xl_of(as_z(s.z) * A + as_z(s.c) * B);
+                s.z = add_n(as_z(s.z) * A, as_z(s.c) * B);
                 s.zs = to_std(s.z);
-                if (a.calc_dc) s.d = xl_of(as_z(s.d) * A);
-                if (a.calc_dz) s.h = xl_of(as_z(s.h) * A);
+                if (a.calc_dc) s.d = mul_n(as_z(s.d), A);
+                if (a.calc_dz) s.h = mul_n(as_z(s.h), A);
                 k.bla++; k.bla_it += st;
             }
         }
         s.n += 1; k.plain++;
         const xz Zr = as_z(ZR(t, s.w));
         const xz z0 = as_z(s.z);
-        if (a.calc_dc) s.d = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.d), Zr, s.dyn ? xz_zero() : DCz(t, s.w)));
-        if (a.calc_dz) s.h = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.h), Zr, s.nul ? DZz(t, 0) : DZz(t, s.w)));
-        s.z = xl_of(p_zn<KIND, DEG>(z0, Zr, as_z(s.c)));
+        if constexpr (KIND == KIND_M2) {
+            // mandelbrot_M2.py:607-622 with Z + z formed once for both
+            // derivatives and the factors 2 taken into the exponents (exact)
+            const xz Zz = Zr + z0;
+            if (a.calc_dc) s.d = add_n(Zz * as_z(s.d), (s.dyn ? xz_zero() : DCz(t, s.w)) * z0, 1);
+            if (a.calc_dz) s.h = add_n(Zz * as_z(s.h), (s.nul ? DZz(t, 0) : DZz(t, s.w)) * z0, 1);
+            s.z = add_n(z0 * (z0 + muls(2., Zr)), as_z(s.c));
+        } else {
+            if (a.calc_dc) s.d = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.d), Zr, s.dyn ? xz_zero() : DCz(t, s.w)));
+            if (a.calc_dz) s.h = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.h), Zr, s.nul ? DZz(t, 0) : DZz(t, s.w)));
+            s.z = xl_of(p_zn<KIND, DEG>(z0, Zr, as_z(s.c)));
+        }
         s.zs = to_std(s.z);
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;
         if (a.calc_dz) {
             const xz nx = s.nul ? DZz(t, 0) : (s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
-            if (abs2_lt(as_z(s.h) + nx, a.eps2)) return 2;
+            if (abs2_lt(add_raw(as_z(s.h), nx), a.eps2)) return 2;
         }
         const cplx ZZ = s.zs + t.Zn[s.w];
         s.orb.record(s.n, a.backshift, ZZ.re, ZZ.im);
         if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
-        if (s.w >= t.div_iter - 1) {
-            s.zs = ZZ;
-            s.z = xl_of(ZZ);
+        // both rebases (perturbation.py:1283-1372) share the derivative
+        // updates: one block, whichever of them a lane takes
+        const bool reb_div = s.w >= t.div_iter - 1;
+        bool reb = reb_div;
+        xl ZZx;
+        if (!reb_div) {
+            s.dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
+            if (s.dyn) {
+                ZZx = s.z + ZR(t, s.w);
+                reb = le_abs2(ZZx, s.z);
+            }
+        }
+        if (reb) {
+            if (reb_div) { s.zs = ZZ; s.z = xl_of(ZZ); }
+            else { s.z = ZZx; s.zs = to_std(ZZx); }
             if (a.calc_dc) s.d = s.d + DC(t, s.w);
             if (a.calc_dz) {
                 if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
@@ -265,22 +286,6 @@ struct HoloXr {
             }
             s.w = 0;
             k.reb++;
-            return -1;
-        }
-        s.dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
-        if (s.dyn) {
-            const xl ZZx = s.z + ZR(t, s.w);
-            if (le(abs2(ZZx), abs2(s.z))) {
-                s.z = ZZx;
-                s.zs = to_std(ZZx);
-                if (a.calc_dc) s.d = s.d + DC(t, s.w);
-                if (a.calc_dz) {
-                    if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
-                    s.nul = true;
-                }
-                s.w = 0;
-                k.reb++;
-            }
         }
         return -1;
     }

@@ -187,7 +187,39 @@ FSG_DEV xz operator*(xz a, cplx b) {
     return xz_mk(ac - bd, ad + bc, a.e);
 }
 FSG_DEV xz muls(double s, xz a) { xz r; r.re = s * a.re; r.im = s * a.im; r.e = a.e; return r; }   // s != 0
+// 2^f (a + b), normalised: the sum of the last operation of an iteration.
+// The normalisation tests the result for zero, so the sum needs no zero
+// test of its own; the factor 2^f goes to the exponent.
+FSG_DEV xl add_n(xz a, xz b, int32_t f = 0) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
+    return xl_n(ldexp(a.re, sa) + ldexp(b.re, sb), ldexp(a.im, sa) + ldexp(b.im, sb), E + f);
+}
+// a b, normalised (the normalisation's zero test replaces the product's)
+FSG_DEV xl mul_n(xz a, cplx b) {
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    return xl_n(ac - bd, ad + bc, a.e);
+}
+// a + b for a sum that only a comparison reads: a zero result needs no
+// exponent of its own there
+FSG_DEV xz add_raw(xz a, xz b) {
+    const int32_t E = max(a.e, b.e);
+    const int32_t sa = shift_of(a.e, E), sb = shift_of(b.e, E);
+    xz r;
+    r.re = ldexp(a.re, sa) + ldexp(b.re, sb); r.im = ldexp(a.im, sa) + ldexp(b.im, sb); r.e = E;
+    return r;
+}
 FSG_DEV xlf abs2(xz a) { return xlf_n(a.re * a.re + a.im * a.im, 2 * max(a.e, -(1 << 27))); }
+// |a|^2 <= |b|^2 for normalised a, b: le(abs2(a), abs2(b)) without
+// normalising the two squares -- each sum of squares lies in [1, 8) (or is
+// zero), so the aligned larger one is at least 1 and a shift that takes the
+// other below the normal range decides the comparison the same way
+FSG_DEV bool le_abs2(xl a, xl b) {
+    const double qa = a.re * a.re + a.im * a.im, qb = b.re * b.re + b.im * b.im;
+    const int32_t ea = 2 * max(a.e, -(1 << 27)), eb = 2 * max(b.e, -(1 << 27));
+    const int32_t E = max(ea, eb);
+    return ldexp(qa, ea - E) <= ldexp(qb, eb - E);
+}
 // |a|^2 < b for a binary64 b > 0: the mantissa sum rounds as abs2's, and
 // scaling it by 2^(2e) is exact unless the result leaves the normal range,
 // where it lies far on one side of b whichever way it rounds
