@@ -63,7 +63,11 @@ __device__ __forceinline__ int64_t grid_slot(int64_t s, int64_t n, int64_t w) {
 // start.  The highest valid stage is therefore found by bisection, with
 // about log2(stages) dependent loads instead of one per stage.
 // Returns the jump length (0 if no stage is valid) and the entry index.
-__device__ __forceinline__ int32_t bla_find(const Tables &t, double az, int32_t w, int32_t &idx) {
+// SCALED: |z| = az 2^ez (the modulus of an extended-range mantissa and its
+// exponent); each radius is scaled by 2^-ez instead, which is exact.
+template <bool SCALED = false>
+__device__ __forceinline__ int32_t bla_find(const Tables &t, double az, int32_t w, int32_t &idx,
+                                            int32_t ez = 0) {
     const int32_t it = w >> 3;
     const int32_t invalid_step = t.first_invalid - w;
     if (invalid_step <= 8) return 0;
@@ -73,7 +77,8 @@ __device__ __forceinline__ int32_t bla_find(const Tables &t, double az, int32_t 
     int32_t lo = 3, best = 0;
     while (lo <= hi) {
         const int32_t mid = (lo + hi) >> 1;
-        if (az < t.r[2 * it + (1 << (mid - 3)) - 1]) { best = mid; lo = mid + 1; }
+        const double r = t.r[2 * it + (1 << (mid - 3)) - 1];
+        if (az < (SCALED ? ldexp(r, -ez) : r)) { best = mid; lo = mid + 1; }
         else hi = mid - 1;
     }
     if (best == 0) return 0;

@@ -92,6 +92,14 @@ __device__ __forceinline__ int32_t bla_lookup(const Tables &t, double zre, doubl
                                               int32_t w, int32_t &idx) {
     return bla_find(t, hypot2(zre, zim), w, idx);
 }
+// The same on an extended-range z: the modulus of its mantissa against the
+// radii scaled by 2^-e.  Where the binary64 copy of z is a normal number
+// this takes the decision hypot2 would (the prescale of hypot2 is the same
+// power of two); where that copy is subnormal or zero it keeps |z|'s
+// precision.
+__device__ __forceinline__ int32_t bla_lookup(const Tables &t, xl z, int32_t w, int32_t &idx) {
+    return bla_find<true>(t, sqrt(z.re * z.re + z.im * z.im), w, idx, z.e);
+}
 
 // ---- pixel -> c (ref_path_c_from_pix + lin_proj_impl, perturbation.py:2214-2230, 2665-2672)
 FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
@@ -198,9 +206,8 @@ template <int KIND, int DEG, bool ORB = false>
 struct HoloXr {
     struct State { xl z, d, h, c; cplx zs; int32_t w, n; bool dyn, nul; OrbitRec<ORB> orb; };
 
-    __device__ static xl DC(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xl_of(m.re, m.im, t.dc_e[i]); }
-    __device__ static xl DZ(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xl_of(m.re, m.im, t.dz_e[i]); }
-    // the same, unnormalised: the host's Xrange records keep |mantissa| within 2^+-100
+    // derivative path values, unnormalised: the host's Xrange records keep
+    // |mantissa| within 2^+-100
     __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xz_mk(m.re, m.im, t.dc_e[i]); }
     __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xz_mk(m.re, m.im, t.dz_e[i]); }
     // orbit point i: the register's Xrange value where the binary64 orbit
@@ -223,7 +230,7 @@ struct HoloXr {
         const Tables &t = a.t;
         while (t.bla_on && (s.w & 7) == 0) {
             int32_t idx;
-            int32_t st = bla_lookup(t, s.zs.re, s.zs.im, s.w, idx);
+            int32_t st = bla_lookup(t, s.z, s.w, idx);
             k.tries++;
             if (st == 0) break;
             {
@@ -279,9 +286,10 @@ struct HoloXr {
         if (reb) {
             if (reb_div) { s.zs = ZZ; s.z = xl_of(ZZ); }
             else { s.z = ZZx; s.zs = to_std(ZZx); }
-            if (a.calc_dc) s.d = s.d + DC(t, s.w);
+            // the path values unnormalised: the sum's normalisation covers them
+            if (a.calc_dc) s.d = add_n(as_z(s.d), DCz(t, s.w));
             if (a.calc_dz) {
-                if (!s.nul) s.h = s.h + (s.n == t.order ? DZ(t, t.L) : DZ(t, s.w));
+                if (!s.nul) s.h = add_n(as_z(s.h), s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
                 s.nul = true;
             }
             s.w = 0;
@@ -299,7 +307,7 @@ struct HoloXr {
         // standard dzndz field keeps the 0 written by numba_initialize
         if (a.calc_dz) { a.Z[2 * (n + i)] = 0.; a.Z[2 * (n + i) + 1] = 0.; f = 2; }
         if (a.calc_dc) {
-            cplx d = to_std(s.d + DC(a.t, s.w));
+            cplx d = to_std(add_n(as_z(s.d), DCz(a.t, s.w)));
             a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
         }
         if constexpr (ORB) orbit_finish_holo<KIND, DEG>(s.orb, a, s.n, to_std(s.c), i);

@@ -632,9 +632,14 @@ static int make_bla(or_ctx *k) {
     return 0;
 }
 
-/* ref_BLA_get, perturbation.py:2116-2170 */
-static inline int64_t bla_get(const or_ctx *k, double zre, double zim, int64_t n,
-                              int64_t first_invalid, const double **Mout) {
+/* ref_BLA_get, perturbation.py:2116-2170: the first stage from the top with
+ * |z| < r.  |z| = az * 2^ez: the binary64 loops pass hypot(z) and ez = 0;
+ * the Xrange loop passes the modulus of its mantissa and its exponent, and
+ * the radius is scaled instead (exact), so that |z| keeps its precision where
+ * the binary64 copy of z (the reference's Z[zn]) is subnormal or zero.  With
+ * a normal copy the two give the same decision. */
+static inline int64_t bla_get_scaled(const or_ctx *k, double az, int32_t ez, int64_t n,
+                                     int64_t first_invalid, const double **Mout) {
     int64_t it = n >> STG_COMPRESSED;
     int stages;
     for (stages = STG_COMPRESSED; stages < k->stages_bla; stages++) {
@@ -644,17 +649,20 @@ static inline int64_t bla_get(const or_ctx *k, double zre, double zim, int64_t n
     if (stages == k->stages_bla) stages = k->stages_bla - 1;
     int64_t invalid_step = first_invalid - n;
     int dim = (k->kind == OR_KIND_BS) ? 8 : 4;
-    double az = fs_hypot(zre, zim);
     for (int stg = stages; stg >= STG_COMPRESSED; stg--) {
         int64_t step = 1LL << stg;
         if (step >= invalid_step) continue;
         int64_t idx = bla_index(n >> STG_COMPRESSED, stg - STG_COMPRESSED);
-        if (az < k->r_bla[idx]) {
+        if (az < ldexp(k->r_bla[idx], -ez)) {
             *Mout = k->M_bla + idx * dim;
             return step;
         }
     }
     return 0;
+}
+static inline int64_t bla_get(const or_ctx *k, double zre, double zim, int64_t n,
+                              int64_t first_invalid, const double **Mout) {
+    return bla_get_scaled(k, fs_hypot(zre, zim), 0, n, first_invalid, Mout);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -858,7 +866,8 @@ static int64_t iterate_holo(const or_ctx *k, xrc c_xr, cpx *out_z, cpx *out_dznd
     xrc out_xr = xzero;
     for (;;) {
         if (k->bla_on && (w & 7) == 0) {
-            step = bla_get(k, zs.re, zs.im, w, first_invalid, &Mb);
+            step = bla_get_scaled(k, sqrt(zx.m.re * zx.m.re + zx.m.im * zx.m.im), zx.e, w,
+                                  first_invalid, &Mb);
             if (step != 0) {
                 cpx A = C(Mb[0], Mb[1]), B = C(Mb[2], Mb[3]);
                 n += step;
