@@ -5,6 +5,7 @@
 // Xrange code as the kernels) and the BLA tree (built on the GPU).
 #include "fsgpu.h"
 #include "models.hpp"
+#include "xl.hpp"
 
 #include <memory>
 #include <mutex>
@@ -48,6 +49,25 @@ template <class T> T *dalloc(size_t n, hipError_t &err) {
     if (n == 0) n = 1;
     err = hipMalloc(&p, n * sizeof(T));
     return static_cast<T *>(p);
+}
+
+// Tables::zn_k of a holomorphic reference: per orbit point, the exponent k
+// that xl_n (xl.hpp) takes to normalise its binary64 value (larger
+// component scaled into [1, 2); the exponent of zero for a zero point), or
+// INT32_MIN + its Xrange register slot
+void point_codes(const fsg_ref_desc *d, std::vector<int32_t> &code) {
+    const int64_t L = d->ref_len;
+    code.assign((size_t)L, 0);
+    for (int64_t i = 0; i < L; i++) {
+        const double mx = std::max(std::fabs(d->Zn_path[2 * i]), std::fabs(d->Zn_path[2 * i + 1]));
+        int k0 = 0;
+        if (mx != 0.) (void)std::frexp(mx, &k0);
+        code[(size_t)i] = (mx == 0.) ? XL_ZERO_E : k0 - 1;
+    }
+    for (int32_t j = 0; j < d->n_xr; j++) {
+        const int64_t i = d->xr_index[j];
+        if (i >= 0 && i < L) code[(size_t)i] = INT32_MIN + j;
+    }
 }
 
 // ---- host-side orbit derivative paths (perturbation.py:2282-2516) ----
@@ -207,7 +227,7 @@ struct fsg_ctx {
     int32_t *dc_e = nullptr, *dz_e = nullptr;
     double *bs[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *bs_e[4] = {nullptr, nullptr, nullptr, nullptr};
-    int32_t *xr_idx = nullptr, *xr_e = nullptr, *xr_pos = nullptr;
+    int32_t *xr_idx = nullptr, *xr_e = nullptr, *xr_pos = nullptr, *zn_k = nullptr;
     double *xr_m = nullptr;
     double *M = nullptr, *r = nullptr;
     int64_t bla_len = 0;
@@ -322,7 +342,10 @@ int fill_args(fsg_ctx *c, KArgs &a) {
     t.dc = c->dc; t.dc_e = c->dc_e; t.dz = c->dz; t.dz_e = c->dz_e;
     for (int q = 0; q < 4; q++) { t.bs[q] = c->bs[q]; t.bs_e[q] = c->bs_e[q]; }
     t.n_xr = c->d.n_xr; t.xr_idx = c->xr_idx; t.xr_m = c->xr_m; t.xr_e = c->xr_e;
-    t.xr_pos = c->d.n_xr ? c->xr_pos : nullptr;
+    // the burning-ship kernels find register slots in xr_pos, the
+    // holomorphic ones in zn_k
+    t.xr_pos = (c->d.kind == FSG_KIND_BS && c->d.n_xr) ? c->xr_pos : nullptr;
+    t.zn_k = c->zn_k;
     t.M = c->M; t.r = c->r; t.stages_bla = c->stages; t.bla_on = c->d.bla_on;
     a.tx = c->tx;
     for (int q = 0; q < 4; q++) a.lin[q] = c->d.lin_mat[q];
@@ -581,6 +604,15 @@ int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
             for (int32_t j = 0; j < nx; j++)
                 if (desc->xr_index[j] >= 0 && desc->xr_index[j] <= L) pos[(size_t)desc->xr_index[j]] = j;
             if ((rc = upload(c, c->xr_pos, pos.data(), pos.size(), true))) return cleanup(rc);
+        }
+    }
+    if (holo) {
+        if (base && base->zn_k) {
+            c->zn_k = base->zn_k;
+        } else {
+            std::vector<int32_t> code;
+            point_codes(desc, code);
+            if ((rc = upload(c, c->zn_k, code.data(), code.size(), true))) return cleanup(rc);
         }
     }
 

@@ -6,6 +6,9 @@ namespace fsg {
 
 enum Kind { KIND_M2 = 0, KIND_MN = 1, KIND_BS = 2 };
 
+// Tables::zn_k codes below this one name an Xrange register slot
+constexpr int32_t ZK_REGISTER = -(1 << 28);
+
 // Read-only per-reference data, resident in HBM for the life of a context.
 struct Tables {
     const cplx *Zn;            // [L] reference orbit Z_n rounded to binary64
@@ -21,6 +24,12 @@ struct Tables {
     int32_t n_xr; int32_t pad0;
     const int32_t *xr_idx; const double *xr_m; const int32_t *xr_e;
     const int32_t *xr_pos;
+    // holomorphic orbit points in the extended-range kernels' normalised
+    // form, one code per point [L]: k >= ZK_REGISTER, the binary64 point
+    // scaled by 2^-k (larger component in [1, 2); k is the exponent of zero
+    // for a zero point); k < ZK_REGISTER, the Xrange register's slot
+    // k - INT32_MIN
+    const int32_t *zn_k;
     // BLA tree: 4 doubles (A, B complex) or 8 doubles (2x2 A, 2x2 B) per entry
     const double *M; const double *r;
     int32_t stages_bla; int32_t bla_on;
@@ -45,7 +54,7 @@ struct TablesX {
 // at [K]; KB BLA entries cover every jump starting below K.
 struct LdsPlan {
     int32_t K, KB, bytes, bla_dim;
-    int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, r, M;
+    int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, zn_k, r, M;
 };
 
 // The binary32 form's tables staged in LDS (byte offsets, -1 = read from

@@ -249,7 +249,7 @@ int max_lds_bytes() {
 // Entries 0 .. K-1 of each staged table (K = the iterations a pixel can
 // reach, min(L, div_iter + 1, order)), with the wrapped dZndz value at [K].
 // The tables are taken in the order the loop reads them most -- orbit,
-// Xrange slots, dZndz, dZndc (burning ship: its four paths), BLA radii and
+// normalisation codes / Xrange slots, dZndz, dZndc (burning ship: its four paths), BLA radii and
 // coefficients -- as long as they fit; bytes = 0 when not even the orbit does.
 LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     LdsPlan p;
@@ -270,6 +270,7 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     const auto take = [&](int64_t bytes) { int64_t o = off; off += al(bytes); return (int32_t)o; };
     if (!group({16 * K})) { p.K = (int32_t)K; p.KB = 0; p.bla_dim = bla_dim; p.bytes = 0; return p; }
     p.zn = take(16 * K);
+    if (t.zn_k && group({4 * K})) p.zn_k = take(4 * K);
     if (t.xr_pos && group({4 * K})) p.xr_pos = take(4 * K);
     if (t.dz && group({16 * (K + 1), t.dz_e ? 4 * (K + 1) : 0})) {
         p.dz = take(16 * (K + 1));
@@ -298,7 +299,7 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
 }
 
 bool lds_plan_full(const Tables &t, const LdsPlan &p) {
-    bool ok = p.zn >= 0 && (!t.xr_pos || p.xr_pos >= 0) && (!t.dz || p.dz >= 0) &&
+    bool ok = p.zn >= 0 && (!t.xr_pos || p.xr_pos >= 0) && (!t.zn_k || p.zn_k >= 0) && (!t.dz || p.dz >= 0) &&
               (!t.dz_e || p.dz_e >= 0) && (!t.dc || p.dc >= 0) && (!t.dc_e || p.dc_e >= 0) &&
               (!t.bla_on || (p.r >= 0 && p.M >= 0));
     for (int q = 0; q < 4; q++) ok = ok && (!t.bs[q] || p.bs[q] >= 0) && (!t.bs_e[q] || p.bs_e[q] >= 0);

@@ -208,14 +208,22 @@ struct HoloXr {
 
     // derivative path values, unnormalised: the host's Xrange records keep
     // |mantissa| within 2^+-100
-    __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xz_mk(m.re, m.im, t.dc_e[i]); }
-    __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xz_mk(m.re, m.im, t.dz_e[i]); }
-    // orbit point i: the register's Xrange value where the binary64 orbit
-    // underflows, the binary64 value elsewhere
+    __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xz_ld(m.re, m.im, t.dc_e[i]); }
+    __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xz_ld(m.re, m.im, t.dz_e[i]); }
+    // orbit point i, normalised: the register's Xrange value where the
+    // binary64 orbit underflows, the binary64 value elsewhere, scaled by the
+    // exponent the tables hold for it (xl_of of the point, without its
+    // exponent extraction)
     __device__ static xl ZR(const Tables &t, int32_t i) {
-        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
-        if (j >= 0) return xl_of(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]);
-        return xl_of(t.Zn[i]);
+        const int32_t k = t.zn_k[i];
+        if (k < ZK_REGISTER) {
+            const int32_t j = k - INT32_MIN;
+            return xl_of(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]);
+        }
+        const cplx z = t.Zn[i];
+        xl r;
+        r.re = ldexp(z.re, -k); r.im = ldexp(z.im, -k); r.e = k;
+        return r;
     }
 
     __device__ static void init(State &s, const KArgs &a, double px, double py) {
