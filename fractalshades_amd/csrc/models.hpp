@@ -61,6 +61,36 @@ FSG_HD T p_deriv(T z, T d, T Z, T Zd) {
     }
 }
 
+// z^d + c and the perturbation of ND derivatives (dz/dc and/or dz/dz) in
+// one pass: p_zn's partial products are the factors "tmp" and "mul" of
+// p_deriv, and the powers of Z are the same, so they are formed once.  Every
+// value is the same expression, in the same order, as in p_zn and p_deriv.
+// dv[i] in: the pixel's derivative delta, out: its next value; Zdv[i]: the
+// reference orbit's value of the same derivative.
+template <int DEG, int ND, class T>
+FSG_HD T p_zn_derivs(T z, T Z, T c, T *dv, const T *Zdv) {
+    T mul = z + muls(cbinom(DEG, 1), Z);
+    T tmp = z * mul;
+    T dt[ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dv[i] + muls(cbinom(DEG, 1), Zdv[i]));
+    T Zpk = Z;
+#pragma unroll
+    for (int q = 2; q < DEG; q++) {
+        T dZpk[ND];
+#pragma unroll
+        for (int i = 0; i < ND; i++) dZpk[i] = muls((double)q, Zpk) * Zdv[i];
+        Zpk = Zpk * Z;
+        mul = tmp + muls(cbinom(DEG, q), Zpk);
+#pragma unroll
+        for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dt[i] + muls(cbinom(DEG, q), dZpk[i]));
+        tmp = z * mul;
+    }
+#pragma unroll
+    for (int i = 0; i < ND; i++) dv[i] = dt[i];
+    return tmp + c;
+}
+
 template <int KIND, int DEG, class T>
 FSG_HD T dfdz(T z) {
     if constexpr (KIND == KIND_M2) {

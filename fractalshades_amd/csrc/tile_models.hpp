@@ -101,6 +101,28 @@ __device__ __forceinline__ int32_t bla_lookup(const Tables &t, xl z, int32_t w, 
     return bla_find<true>(t, sqrt(z.re * z.re + z.im * z.im), w, idx, z.e);
 }
 
+// The z^d + c step with the derivatives the launch computes (dz/dc when dc,
+// dz/dz when dz): one pass over the shared partial products
+// (p_zn_derivs); Zd / Zh are read only for the derivatives computed.
+template <int DEG, class T>
+__device__ __forceinline__ T mn_step(T z, T Z, T c, bool dc, bool dz, T &d, T &h, T Zd, T Zh) {
+    if (dc && dz) {
+        T v[2] = {d, h};
+        const T Zv[2] = {Zd, Zh};
+        const T r = p_zn_derivs<DEG, 2>(z, Z, c, v, Zv);
+        d = v[0]; h = v[1];
+        return r;
+    }
+    if (dc || dz) {
+        T v[1] = {dc ? d : h};
+        const T Zv[1] = {dc ? Zd : Zh};
+        const T r = p_zn_derivs<DEG, 1>(z, Z, c, v, Zv);
+        (dc ? d : h) = v[0];
+        return r;
+    }
+    return p_zn<KIND_MN, DEG>(z, Z, c);
+}
+
 // ---- pixel -> c (ref_path_c_from_pix + lin_proj_impl, perturbation.py:2214-2230, 2665-2672)
 FSG_HD xc c_from_pix(const KArgs &a, double px, double py) {
     double x1 = a.lin[0] * px + a.lin[1] * py;
@@ -151,9 +173,15 @@ struct HoloStd {
         }
         s.n += 1; k.plain++;
         const cplx Zr = t.Zn[s.w];
-        if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zc() : t.dc[s.w]);
-        if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? t.dz[0] : t.dz[s.w]);
-        s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+        if constexpr (KIND == KIND_MN) {
+            const cplx Zd = a.calc_dc ? (s.dyn ? zc() : t.dc[s.w]) : zc();
+            const cplx Zh = a.calc_dz ? (s.nul ? t.dz[0] : t.dz[s.w]) : zc();
+            s.z = mn_step<DEG>(s.z, Zr, s.c, a.calc_dc, a.calc_dz, s.d, s.h, Zd, Zh);
+        } else {
+            if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zc() : t.dc[s.w]);
+            if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? t.dz[0] : t.dz[s.w]);
+            s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
+        }
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;
@@ -264,9 +292,12 @@ struct HoloXr {
             if (a.calc_dz) s.h = add_n(Zz * as_z(s.h), (s.nul ? DZz(t, 0) : DZz(t, s.w)) * z0, 1);
             s.z = add_n(z0 * (z0 + muls(2., Zr)), as_z(s.c));
         } else {
-            if (a.calc_dc) s.d = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.d), Zr, s.dyn ? xz_zero() : DCz(t, s.w)));
-            if (a.calc_dz) s.h = xl_of(p_deriv<KIND, DEG>(z0, as_z(s.h), Zr, s.nul ? DZz(t, 0) : DZz(t, s.w)));
-            s.z = xl_of(p_zn<KIND, DEG>(z0, Zr, as_z(s.c)));
+            const xz Zd = a.calc_dc ? (s.dyn ? xz_zero() : DCz(t, s.w)) : xz_zero();
+            const xz Zh = a.calc_dz ? (s.nul ? DZz(t, 0) : DZz(t, s.w)) : xz_zero();
+            xz d = as_z(s.d), h = as_z(s.h);
+            s.z = xl_of(mn_step<DEG>(z0, Zr, as_z(s.c), a.calc_dc, a.calc_dz, d, h, Zd, Zh));
+            if (a.calc_dc) s.d = xl_of(d);
+            if (a.calc_dz) s.h = xl_of(h);
         }
         s.zs = to_std(s.z);
         s.w += 1;
