@@ -65,30 +65,40 @@ FSG_HD T p_deriv(T z, T d, T Z, T Zd) {
 // one pass: p_zn's partial products are the factors "tmp" and "mul" of
 // p_deriv, and the powers of Z are the same, so they are formed once.  Every
 // value is the same expression, in the same order, as in p_zn and p_deriv.
-// dv[i] in: the pixel's derivative delta, out: its next value; Zdv[i]: the
-// reference orbit's value of the same derivative.
-template <int DEG, int ND, class T>
-FSG_HD T p_zn_derivs(T z, T Z, T c, T *dv, const T *Zdv) {
+// dv[i]: the pixel's derivative delta, Zdv[i]: the reference orbit's value
+// of the same derivative, out[i]: the next value of dv[i].  The last sum of
+// each result is taken by add(a, b), which may return another type R (a
+// normalised value).  Returns the next z.
+template <int DEG, int ND, class T, class R, class Add>
+FSG_HD R p_zn_derivs(T z, T Z, T c, const T *dv, const T *Zdv, R *out, Add add) {
     T mul = z + muls(cbinom(DEG, 1), Z);
     T tmp = z * mul;
-    T dt[ND];
+    T dt[ND > 0 ? ND : 1];
+    if constexpr (DEG == 2) {
 #pragma unroll
-    for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dv[i] + muls(cbinom(DEG, 1), Zdv[i]));
+        for (int i = 0; i < ND; i++) out[i] = add(dv[i] * mul, z * (dv[i] + muls(cbinom(DEG, 1), Zdv[i])));
+    } else {
+#pragma unroll
+        for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dv[i] + muls(cbinom(DEG, 1), Zdv[i]));
+    }
     T Zpk = Z;
 #pragma unroll
     for (int q = 2; q < DEG; q++) {
-        T dZpk[ND];
+        T dZpk[ND > 0 ? ND : 1];
 #pragma unroll
         for (int i = 0; i < ND; i++) dZpk[i] = muls((double)q, Zpk) * Zdv[i];
         Zpk = Zpk * Z;
         mul = tmp + muls(cbinom(DEG, q), Zpk);
+        if (q + 1 < DEG) {
 #pragma unroll
-        for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dt[i] + muls(cbinom(DEG, q), dZpk[i]));
+            for (int i = 0; i < ND; i++) dt[i] = dv[i] * mul + z * (dt[i] + muls(cbinom(DEG, q), dZpk[i]));
+        } else {
+#pragma unroll
+            for (int i = 0; i < ND; i++) out[i] = add(dv[i] * mul, z * (dt[i] + muls(cbinom(DEG, q), dZpk[i])));
+        }
         tmp = z * mul;
     }
-#pragma unroll
-    for (int i = 0; i < ND; i++) dv[i] = dt[i];
-    return tmp + c;
+    return add(tmp, c);
 }
 
 template <int KIND, int DEG, class T>

@@ -104,23 +104,30 @@ __device__ __forceinline__ int32_t bla_lookup(const Tables &t, xl z, int32_t w, 
 // The z^d + c step with the derivatives the launch computes (dz/dc when dc,
 // dz/dz when dz): one pass over the shared partial products
 // (p_zn_derivs); Zd / Zh are read only for the derivatives computed.
-template <int DEG, class T>
-__device__ __forceinline__ T mn_step(T z, T Z, T c, bool dc, bool dz, T &d, T &h, T Zd, T Zh) {
+// add(a, b) takes the last sum of each result and returns it as an R
+// (binary64: the sum; extended range: the normalised sum).
+template <int DEG, class T, class R, class Add>
+__device__ __forceinline__ R mn_step(T z, T Z, T c, bool dc, bool dz, R &d, R &h, T d0, T h0, T Zd, T Zh,
+                                     Add add) {
     if (dc && dz) {
-        T v[2] = {d, h};
+        const T v[2] = {d0, h0};
         const T Zv[2] = {Zd, Zh};
-        const T r = p_zn_derivs<DEG, 2>(z, Z, c, v, Zv);
-        d = v[0]; h = v[1];
+        R o[2];
+        const R r = p_zn_derivs<DEG, 2>(z, Z, c, v, Zv, o, add);
+        d = o[0]; h = o[1];
         return r;
     }
     if (dc || dz) {
-        T v[1] = {dc ? d : h};
+        const T v[1] = {dc ? d0 : h0};
         const T Zv[1] = {dc ? Zd : Zh};
-        const T r = p_zn_derivs<DEG, 1>(z, Z, c, v, Zv);
-        (dc ? d : h) = v[0];
+        R o[1];
+        const R r = p_zn_derivs<DEG, 1>(z, Z, c, v, Zv, o, add);
+        (dc ? d : h) = o[0];
         return r;
     }
-    return p_zn<KIND_MN, DEG>(z, Z, c);
+    const T zz[1] = {z};
+    R o[1];
+    return p_zn_derivs<DEG, 0>(z, Z, c, zz, zz, o, add);
 }
 
 // ---- pixel -> c (ref_path_c_from_pix + lin_proj_impl, perturbation.py:2214-2230, 2665-2672)
@@ -176,7 +183,8 @@ struct HoloStd {
         if constexpr (KIND == KIND_MN) {
             const cplx Zd = a.calc_dc ? (s.dyn ? zc() : t.dc[s.w]) : zc();
             const cplx Zh = a.calc_dz ? (s.nul ? t.dz[0] : t.dz[s.w]) : zc();
-            s.z = mn_step<DEG>(s.z, Zr, s.c, a.calc_dc, a.calc_dz, s.d, s.h, Zd, Zh);
+            s.z = mn_step<DEG>(s.z, Zr, s.c, a.calc_dc, a.calc_dz, s.d, s.h, s.d, s.h, Zd, Zh,
+                               [](cplx u, cplx v) { return u + v; });
         } else {
             if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zc() : t.dc[s.w]);
             if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? t.dz[0] : t.dz[s.w]);
@@ -294,10 +302,9 @@ struct HoloXr {
         } else {
             const xz Zd = a.calc_dc ? (s.dyn ? xz_zero() : DCz(t, s.w)) : xz_zero();
             const xz Zh = a.calc_dz ? (s.nul ? DZz(t, 0) : DZz(t, s.w)) : xz_zero();
-            xz d = as_z(s.d), h = as_z(s.h);
-            s.z = xl_of(mn_step<DEG>(z0, Zr, as_z(s.c), a.calc_dc, a.calc_dz, d, h, Zd, Zh));
-            if (a.calc_dc) s.d = xl_of(d);
-            if (a.calc_dz) s.h = xl_of(h);
+            // the last sums normalised at once (add_n: no zero test of their own)
+            s.z = mn_step<DEG>(z0, Zr, as_z(s.c), a.calc_dc, a.calc_dz, s.d, s.h, as_z(s.d), as_z(s.h),
+                               Zd, Zh, [](xz u, xz v) { return add_n(u, v); });
         }
         s.zs = to_std(s.z);
         s.w += 1;
