@@ -106,7 +106,8 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
 // (fsg_lds.hip, 1024-lane blocks)
 typedef void (*tile_fn)(const KArgs);
 tile_fn select_tile_orbit(const LaunchCfg &cfg);
-tile_fn select_tile_lds(const LaunchCfg &cfg, bool full);
+// der: the derivatives the launch computes (bit 0 dz/dc, bit 1 dz/dz)
+tile_fn select_tile_lds(const LaunchCfg &cfg, bool full, int der);
 int tile_lds_block(const LaunchCfg &cfg);        // lanes per block of select_tile_lds's kernel
 // true when the plan stages every table the launch reads
 bool lds_plan_full(const Tables &t, const LdsPlan &p);

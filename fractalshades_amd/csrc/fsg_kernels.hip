@@ -313,7 +313,8 @@ hipError_t launch_tile(const LaunchCfg &cfg, const KArgs &a, hipStream_t stream)
     if (!cfg.orbit && !no_lds) {
         KArgs al = a;
         al.lds = plan_lds(a.t, cfg.kind == KIND_BS ? 8 : 4, max_lds_bytes());
-        tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg, lds_plan_full(a.t, al.lds)) : nullptr;
+        const int der = (a.calc_dc ? 1 : 0) | (a.calc_dz ? 2 : 0);
+        tile_fn f = al.lds.bytes > 0 ? select_tile_lds(cfg, lds_plan_full(a.t, al.lds), der) : nullptr;
         const int block = tile_lds_block(cfg);
         const int fill = f ? fill_blocks(reinterpret_cast<const void *>(f), block, al.lds.bytes) : -1;
         if (fill > 0) {

@@ -16,7 +16,18 @@ template <class Model> static tile_fn fn_lds512(bool full) {
 
 int tile_lds_block(const LaunchCfg &c) { return (c.kind == KIND_MN && c.xr && c.degree >= 6) ? 512 : 1024; }
 
-tile_fn select_tile_lds(const LaunchCfg &c, bool full) {
+// z^2 + c in extended range, the headline case: one instantiation per
+// combination of derivatives (der: bit 0 dz/dc, bit 1 dz/dz)
+static tile_fn fn_lds_m2xr(bool full, int der) {
+    switch (der) {
+        case 0: return fn_lds<HoloXr<KIND_M2, 2, false, 0>>(full);
+        case 1: return fn_lds<HoloXr<KIND_M2, 2, false, 1>>(full);
+        case 2: return fn_lds<HoloXr<KIND_M2, 2, false, 2>>(full);
+        default: return fn_lds<HoloXr<KIND_M2, 2, false, 3>>(full);
+    }
+}
+
+tile_fn select_tile_lds(const LaunchCfg &c, bool full, int der) {
     if (tile_lds_block(c) == 512) {
         switch (c.degree) {
             case 6: return fn_lds512<HoloXr<KIND_MN, 6>>(full);
@@ -25,7 +36,7 @@ tile_fn select_tile_lds(const LaunchCfg &c, bool full) {
             default: return nullptr;
         }
     }
-    if (c.kind == KIND_M2) return c.xr ? fn_lds<HoloXr<KIND_M2, 2>>(full) : fn_lds<HoloStd<KIND_M2, 2>>(full);
+    if (c.kind == KIND_M2) return c.xr ? fn_lds_m2xr(full, der) : fn_lds<HoloStd<KIND_M2, 2>>(full);
     if (c.kind == KIND_MN) {
         switch (c.degree) {
 #define FSG_MN(D) case D: return c.xr ? fn_lds<HoloXr<KIND_MN, D>>(full) : fn_lds<HoloStd<KIND_MN, D>>(full);

@@ -238,8 +238,14 @@ struct HoloStd {
 // the lazily normalised extended-range form of xl.hpp: every result is
 // renormalised without branches, which rounds exactly like the reference's
 // Xrange record (a rescaling by a power of two is exact).
-template <int KIND, int DEG, bool ORB = false>
+// DER >= 0 fixes the derivatives at compile time (bit 0: dz/dc, bit 1:
+// dz/dz): the step is then one block of straight-line code, which the
+// compiler schedules across the z, dz/dc and dz/dz recurrences; DER = -1
+// reads them from the launch arguments.
+template <int KIND, int DEG, bool ORB = false, int DER = -1>
 struct HoloXr {
+    __device__ static bool DC_ON(const KArgs &a) { if constexpr (DER >= 0) return (DER & 1) != 0; else return a.calc_dc != 0; }
+    __device__ static bool DZ_ON(const KArgs &a) { if constexpr (DER >= 0) return (DER & 2) != 0; else return a.calc_dz != 0; }
     struct State { xl z, d, h, c; cplx zs; int32_t w, n; bool dyn, nul; OrbitRec<ORB> orb; };
 
     // derivative path values, unnormalised: the host's Xrange records keep
@@ -284,8 +290,8 @@ struct HoloXr {
                 s.w += st;
                 s.z = add_n(as_z(s.z) * A, as_z(s.c) * B);
                 s.zs = to_std(s.z);
-                if (a.calc_dc) s.d = mul_n(as_z(s.d), A);
-                if (a.calc_dz) s.h = mul_n(as_z(s.h), A);
+                if (DC_ON(a)) s.d = mul_n(as_z(s.d), A);
+                if (DZ_ON(a)) s.h = mul_n(as_z(s.h), A);
                 k.bla++; k.bla_it += st;
             }
         }
@@ -296,21 +302,21 @@ struct HoloXr {
             // mandelbrot_M2.py:607-622 with Z + z formed once for both
             // derivatives and the factors 2 taken into the exponents (exact)
             const xz Zz = Zr + z0;
-            if (a.calc_dc) s.d = add_n(Zz * as_z(s.d), (s.dyn ? xz_zero() : DCz(t, s.w)) * z0, 1);
-            if (a.calc_dz) s.h = add_n(Zz * as_z(s.h), (s.nul ? DZz(t, 0) : DZz(t, s.w)) * z0, 1);
+            if (DC_ON(a)) s.d = add_n(Zz * as_z(s.d), (s.dyn ? xz_zero() : DCz(t, s.w)) * z0, 1);
+            if (DZ_ON(a)) s.h = add_n(Zz * as_z(s.h), (s.nul ? DZz(t, 0) : DZz(t, s.w)) * z0, 1);
             s.z = add_n(z0 * (z0 + muls(2., Zr)), as_z(s.c));
         } else {
-            const xz Zd = a.calc_dc ? (s.dyn ? xz_zero() : DCz(t, s.w)) : xz_zero();
-            const xz Zh = a.calc_dz ? (s.nul ? DZz(t, 0) : DZz(t, s.w)) : xz_zero();
+            const xz Zd = DC_ON(a) ? (s.dyn ? xz_zero() : DCz(t, s.w)) : xz_zero();
+            const xz Zh = DZ_ON(a) ? (s.nul ? DZz(t, 0) : DZz(t, s.w)) : xz_zero();
             // the last sums normalised at once (add_n: no zero test of their own)
-            s.z = mn_step<DEG>(z0, Zr, as_z(s.c), a.calc_dc, a.calc_dz, s.d, s.h, as_z(s.d), as_z(s.h),
+            s.z = mn_step<DEG>(z0, Zr, as_z(s.c), DC_ON(a), DZ_ON(a), s.d, s.h, as_z(s.d), as_z(s.h),
                                Zd, Zh, [](xz u, xz v) { return add_n(u, v); });
         }
         s.zs = to_std(s.z);
         s.w += 1;
         if (s.w >= t.order) s.w = 0;
         if (s.n >= a.max_iter) return 0;
-        if (a.calc_dz) {
+        if (DZ_ON(a)) {
             const xz nx = s.nul ? DZz(t, 0) : (s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
             if (abs2_lt(add_raw(as_z(s.h), nx), a.eps2)) return 2;
         }
@@ -333,8 +339,8 @@ struct HoloXr {
             if (reb_div) { s.zs = ZZ; s.z = xl_of(ZZ); }
             else { s.z = ZZx; s.zs = to_std(ZZx); }
             // the path values unnormalised: the sum's normalisation covers them
-            if (a.calc_dc) s.d = add_n(as_z(s.d), DCz(t, s.w));
-            if (a.calc_dz) {
+            if (DC_ON(a)) s.d = add_n(as_z(s.d), DCz(t, s.w));
+            if (DZ_ON(a)) {
                 if (!s.nul) s.h = add_n(as_z(s.h), s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
                 s.nul = true;
             }
@@ -351,8 +357,8 @@ struct HoloXr {
         int f = 1;
         // perturbation.py:1377-1380 writes zn and dzndc back only; the
         // standard dzndz field keeps the 0 written by numba_initialize
-        if (a.calc_dz) { a.Z[2 * (n + i)] = 0.; a.Z[2 * (n + i) + 1] = 0.; f = 2; }
-        if (a.calc_dc) {
+        if (DZ_ON(a)) { a.Z[2 * (n + i)] = 0.; a.Z[2 * (n + i) + 1] = 0.; f = 2; }
+        if (DC_ON(a)) {
             cplx d = to_std(add_n(as_z(s.d), DCz(a.t, s.w)));
             a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
         }

@@ -31,6 +31,27 @@ def test_m2_bitexact(name, xr):
     assert stats["plain_steps"] > 0
 
 
+@pytest.mark.parametrize("dc,dz", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_m2_xr_derivative_combinations_bitexact(dc, dz):
+    # the extended-range z^2 + c kernel is compiled once per combination of
+    # derivatives: each one against the oracle, on a strided subsample of the
+    # interior-dominated workload (2048x2048 at 5e-601, max_iter 2e5), where
+    # pixels stop on all three tests
+    if not _cached("C2_int"):
+        pytest.skip("no cached seeded centre for C2_int")
+    f = configs.build(configs.WORKLOADS["C2_int"])
+    args = with_fields(f._calc_data["bench"]["cycle_indep_args"], calc_dzndc=dc, interior_detect=dz)
+    c_pix = np.ravel(f.chunk_pixel_pos((0, f.nx, 0, f.ny)))
+    c_pix = np.ascontiguousarray(c_pix[np.arange(0, c_pix.size, 2039)])
+    print(f"case C2_int dc={dc} dz={dz} npts={c_pix.size}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+    assert stats["bla_steps"] > 0 and stats["rebases"] > 0
+    hist = np.bincount(ref[2].ravel().astype(np.int64), minlength=3)
+    assert hist[0] > 0 and hist[1] > 0 and (hist[2] > 0) == bool(dz)
+
+
 @pytest.mark.parametrize("bla", [0, 1])
 def test_m2_bla_switch_bitexact(bla):
     f = frames.make("M2_E20", nx=64)
