@@ -51,22 +51,31 @@ template <class T> T *dalloc(size_t n, hipError_t &err) {
     return static_cast<T *>(p);
 }
 
-// Tables::zn_k of a holomorphic reference: per orbit point, the exponent k
-// that xl_n (xl.hpp) takes to normalise its binary64 value (larger
-// component scaled into [1, 2); the exponent of zero for a zero point), or
-// INT32_MIN + its Xrange register slot
-void point_codes(const fsg_ref_desc *d, std::vector<int32_t> &code) {
+// Tables::zn_k: per orbit point, the exponent k that xl_n / xlf_n (xl.hpp)
+// take to normalise its binary64 value (larger component, or the one
+// component, scaled into [1, 2); the exponent of zero for a zero value), or
+// INT32_MIN + its Xrange register slot.  Holomorphic: one code per point
+// (the complex value); burning ship: two (x, then y).
+void point_codes(const fsg_ref_desc *d, int w, std::vector<int32_t> &code) {
     const int64_t L = d->ref_len;
-    code.assign((size_t)L, 0);
-    for (int64_t i = 0; i < L; i++) {
-        const double mx = std::max(std::fabs(d->Zn_path[2 * i]), std::fabs(d->Zn_path[2 * i + 1]));
+    code.assign((size_t)(w * L), 0);
+    const auto norm_exp = [](double mx) {
         int k0 = 0;
         if (mx != 0.) (void)std::frexp(mx, &k0);
-        code[(size_t)i] = (mx == 0.) ? XL_ZERO_E : k0 - 1;
+        return (mx == 0.) ? XL_ZERO_E : k0 - 1;
+    };
+    for (int64_t i = 0; i < L; i++) {
+        const double x = std::fabs(d->Zn_path[2 * i]), y = std::fabs(d->Zn_path[2 * i + 1]);
+        if (w == 1) {
+            code[(size_t)i] = norm_exp(std::max(x, y));
+        } else {
+            code[(size_t)(2 * i)] = norm_exp(x);
+            code[(size_t)(2 * i + 1)] = norm_exp(y);
+        }
     }
     for (int32_t j = 0; j < d->n_xr; j++) {
         const int64_t i = d->xr_index[j];
-        if (i >= 0 && i < L) code[(size_t)i] = INT32_MIN + j;
+        if (i >= 0 && i < L) code[(size_t)(w * i)] = INT32_MIN + j;
     }
 }
 
@@ -227,7 +236,8 @@ struct fsg_ctx {
     int32_t *dc_e = nullptr, *dz_e = nullptr;
     double *bs[4] = {nullptr, nullptr, nullptr, nullptr};
     int32_t *bs_e[4] = {nullptr, nullptr, nullptr, nullptr};
-    int32_t *xr_idx = nullptr, *xr_e = nullptr, *xr_pos = nullptr, *zn_k = nullptr;
+    int32_t *xr_idx = nullptr, *xr_e = nullptr, *zn_k = nullptr;
+    int32_t zn_k_w = 0;
     double *xr_m = nullptr;
     double *M = nullptr, *r = nullptr;
     int64_t bla_len = 0;
@@ -342,10 +352,7 @@ int fill_args(fsg_ctx *c, KArgs &a) {
     t.dc = c->dc; t.dc_e = c->dc_e; t.dz = c->dz; t.dz_e = c->dz_e;
     for (int q = 0; q < 4; q++) { t.bs[q] = c->bs[q]; t.bs_e[q] = c->bs_e[q]; }
     t.n_xr = c->d.n_xr; t.xr_idx = c->xr_idx; t.xr_m = c->xr_m; t.xr_e = c->xr_e;
-    // the burning-ship kernels find register slots in xr_pos, the
-    // holomorphic ones in zn_k
-    t.xr_pos = (c->d.kind == FSG_KIND_BS && c->d.n_xr) ? c->xr_pos : nullptr;
-    t.zn_k = c->zn_k;
+    t.zn_k = c->zn_k; t.zn_k_w = c->zn_k_w;
     t.M = c->M; t.r = c->r; t.stages_bla = c->stages; t.bla_on = c->d.bla_on;
     a.tx = c->tx;
     for (int q = 0; q < 4; q++) a.lin[q] = c->d.lin_mat[q];
@@ -591,29 +598,22 @@ int ctx_create(const fsg_ref_desc *desc, fsg_ctx *base, fsg_ctx **out) {
     if (base) {
         c->shared = base->shared;
         c->Zn = base->Zn; c->xr_idx = base->xr_idx; c->xr_m = base->xr_m; c->xr_e = base->xr_e;
-        c->xr_pos = base->xr_pos;
     } else {
         c->shared = std::make_shared<fsg_ctx::Shared>();
         if ((rc = upload(c, c->Zn, reinterpret_cast<const cplx *>(desc->Zn_path), (size_t)L, true))) return cleanup(rc);
         if ((rc = upload(c, c->xr_idx, desc->xr_index, (size_t)nx, true))) return cleanup(rc);
         if ((rc = upload(c, c->xr_m, desc->xr_mant, (size_t)2 * nx, true))) return cleanup(rc);
         if ((rc = upload(c, c->xr_e, desc->xr_exp, (size_t)(holo ? nx : 2 * nx), true))) return cleanup(rc);
-        if (nx) {
-            // dense slot table: one load per orbit point instead of a search
-            std::vector<int32_t> pos((size_t)L + 1, -1);
-            for (int32_t j = 0; j < nx; j++)
-                if (desc->xr_index[j] >= 0 && desc->xr_index[j] <= L) pos[(size_t)desc->xr_index[j]] = j;
-            if ((rc = upload(c, c->xr_pos, pos.data(), pos.size(), true))) return cleanup(rc);
-        }
     }
-    if (holo) {
-        if (base && base->zn_k) {
-            c->zn_k = base->zn_k;
-        } else {
-            std::vector<int32_t> code;
-            point_codes(desc, code);
-            if ((rc = upload(c, c->zn_k, code.data(), code.size(), true))) return cleanup(rc);
-        }
+    // the normalisation codes: one load per orbit point, also where the
+    // point comes from the register (no search of the sorted indices)
+    c->zn_k_w = holo ? 1 : 2;
+    if (base && base->zn_k && base->zn_k_w == c->zn_k_w) {
+        c->zn_k = base->zn_k;
+    } else {
+        std::vector<int32_t> code;
+        point_codes(desc, c->zn_k_w, code);
+        if ((rc = upload(c, c->zn_k, code.data(), code.size(), true))) return cleanup(rc);
     }
 
     HostRef h{&c->d, c->L, c->div_iter, c->order};

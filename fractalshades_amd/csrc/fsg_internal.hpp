@@ -19,16 +19,16 @@ struct Tables {
     const cplx *dz; const int32_t *dz_e;
     // Burning-ship Jacobian paths dX/da, dX/db, dY/da, dY/db [L]
     const double *bs[4]; const int32_t *bs_e[4];
-    // sparse Xrange register of orbit points below 1e-300 (sorted indices);
-    // xr_pos[i] = register slot of orbit point i or -1 (null when empty)
-    int32_t n_xr; int32_t pad0;
+    // sparse Xrange register of orbit points below 1e-300 (sorted indices)
+    int32_t n_xr; int32_t zn_k_w;
     const int32_t *xr_idx; const double *xr_m; const int32_t *xr_e;
-    const int32_t *xr_pos;
-    // holomorphic orbit points in the extended-range kernels' normalised
-    // form, one code per point [L]: k >= ZK_REGISTER, the binary64 point
-    // scaled by 2^-k (larger component in [1, 2); k is the exponent of zero
-    // for a zero point); k < ZK_REGISTER, the Xrange register's slot
-    // k - INT32_MIN
+    // orbit points in the extended-range kernels' normalised form, zn_k_w
+    // codes per point [zn_k_w L]: holomorphic, one per point (the complex
+    // value); burning ship, two (x, then y).  A code k >= ZK_REGISTER: the
+    // binary64 value scaled by 2^-k is the normalised mantissa (larger
+    // component in [1, 2); k is the exponent of zero for a zero value);
+    // k < ZK_REGISTER (in the first code of a point): the point is the
+    // Xrange register's slot k - INT32_MIN
     const int32_t *zn_k;
     // BLA tree: 4 doubles (A, B complex) or 8 doubles (2x2 A, 2x2 B) per entry
     const double *M; const double *r;
@@ -54,7 +54,7 @@ struct TablesX {
 // at [K]; KB BLA entries cover every jump starting below K.
 struct LdsPlan {
     int32_t K, KB, bytes, bla_dim;
-    int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], xr_pos, zn_k, r, M;
+    int32_t zn, dc, dc_e, dz, dz_e, bs[4], bs_e[4], zn_k, r, M;
 };
 
 // The binary32 form's tables staged in LDS (byte offsets, -1 = read from

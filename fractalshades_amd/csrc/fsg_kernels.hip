@@ -270,8 +270,7 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
     const auto take = [&](int64_t bytes) { int64_t o = off; off += al(bytes); return (int32_t)o; };
     if (!group({16 * K})) { p.K = (int32_t)K; p.KB = 0; p.bla_dim = bla_dim; p.bytes = 0; return p; }
     p.zn = take(16 * K);
-    if (t.zn_k && group({4 * K})) p.zn_k = take(4 * K);
-    if (t.xr_pos && group({4 * K})) p.xr_pos = take(4 * K);
+    if (t.zn_k && group({4 * K * t.zn_k_w})) p.zn_k = take(4 * K * t.zn_k_w);
     if (t.dz && group({16 * (K + 1), t.dz_e ? 4 * (K + 1) : 0})) {
         p.dz = take(16 * (K + 1));
         if (t.dz_e) p.dz_e = take(4 * (K + 1));
@@ -299,7 +298,7 @@ LdsPlan plan_lds(const Tables &t, int bla_dim, int max_bytes) {
 }
 
 bool lds_plan_full(const Tables &t, const LdsPlan &p) {
-    bool ok = p.zn >= 0 && (!t.xr_pos || p.xr_pos >= 0) && (!t.zn_k || p.zn_k >= 0) && (!t.dz || p.dz >= 0) &&
+    bool ok = p.zn >= 0 && (!t.zn_k || p.zn_k >= 0) && (!t.dz || p.dz >= 0) &&
               (!t.dz_e || p.dz_e >= 0) && (!t.dc || p.dc >= 0) && (!t.dc_e || p.dc_e >= 0) &&
               (!t.bla_on || (p.r >= 0 && p.M >= 0));
     for (int q = 0; q < 4; q++) ok = ok && (!t.bs[q] || p.bs[q] >= 0) && (!t.bs_e[q] || p.bs_e[q] >= 0);

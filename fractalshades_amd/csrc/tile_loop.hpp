@@ -237,8 +237,7 @@ __global__ void __launch_bounds__(BLOCK) k_tile_lds(const KArgs a) {
         t.bs[q] = stage<FULL>(lds, p.bs[q], a.t.bs[q], K);
         t.bs_e[q] = stage<FULL>(lds, p.bs_e[q], a.t.bs_e[q], K);
     }
-    t.xr_pos = stage<FULL>(lds, p.xr_pos, a.t.xr_pos, K);
-    t.zn_k = stage<FULL>(lds, p.zn_k, a.t.zn_k, K);
+    t.zn_k = stage<FULL>(lds, p.zn_k, a.t.zn_k, K * a.t.zn_k_w);
     t.r = stage<FULL>(lds, p.r, a.t.r, KB);
     t.M = stage<FULL>(lds, p.M, a.t.M, KB * p.bla_dim);
     if ((FULL || p.dz >= 0) && a.t.dz) t.L = (int32_t)K;   // the wrapped dZndz value is staged at [K]

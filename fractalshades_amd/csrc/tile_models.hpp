@@ -453,10 +453,19 @@ struct BsXr {
     struct State { xlf x, y, P[4], a, b; double xs, ys; int32_t w, n; bool dyn; OrbitRec<ORB> orb; };
 
     __device__ static xlf PATH(const Tables &t, int q, int32_t i) { return xlf_of(t.bs[q][i], t.bs_e[q][i]); }
+    // orbit point i, normalised per coordinate: the register's Xrange
+    // values where the binary64 orbit underflows, the binary64 values
+    // elsewhere, scaled by the exponents the tables hold for them
     __device__ static void ZR(const Tables &t, int32_t i, xlf &X, xlf &Y) {
-        const int32_t j = t.xr_pos ? t.xr_pos[i] : -1;
-        if (j >= 0) { X = xlf_of(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xlf_of(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]); }
-        else { const cplx R = t.Zn[i]; X = xlf_of(R.re, 0); Y = xlf_of(R.im, 0); }
+        const int32_t kx = t.zn_k[2 * i], ky = t.zn_k[2 * i + 1];
+        if (kx < ZK_REGISTER) {
+            const int32_t j = kx - INT32_MIN;
+            X = xlf_of(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xlf_of(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]);
+        } else {
+            const cplx R = t.Zn[i];
+            X.m = ldexp(R.re, -kx); X.e = kx;
+            Y.m = ldexp(R.im, -ky); Y.e = ky;
+        }
     }
 
     __device__ static void init(State &s, const KArgs &a, double px, double py) {
