@@ -133,6 +133,7 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False, exte
     specs = serving.serve_requests(n_requests, len(centres), seed=seed)
     bases = {}
     reqs = []
+    widths = [sp.nx for sp in specs]        # the pixels are rows of nx: 8x8 blocks
     posts = [] if layers else None
     for sp in specs:
         f, cia, c_pix = serving.build_request(sp, centres, precision)
@@ -149,11 +150,11 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False, exte
     tiles = serving.layer_tiles(reqs) if layers else serving.pinned_tiles(reqs)
     srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
     try:
-        burst = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts)
+        burst = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts, widths=widths)
         t_first = min(r["t_submit_ms"] for r in burst)
         makespan_s = (max(r["t_done_ms"] for r in burst) - t_first) * 1e-3
         rate = rate_frac * n_requests / makespan_s
-        steady = serving.run_stream(srv, reqs, rate, seed=seed + 1, tiles=tiles, posts=posts)
+        steady = serving.run_stream(srv, reqs, rate, seed=seed + 1, tiles=tiles, posts=posts, widths=widths)
     finally:
         srv.close()
 
@@ -229,7 +230,8 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
         tiles = serving.layer_tiles(reqs)
         srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
         try:
-            rs = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts)
+            rs = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts,
+                                    widths=[sp.nx for sp in specs])
         finally:
             srv.close()
         span = (max(r["t_done_ms"] for r in rs) - min(r["t_submit_ms"] for r in rs)) * 1e-3

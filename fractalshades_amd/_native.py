@@ -184,12 +184,12 @@ def load(path=None):
             "fsg_server_destroy": ([vp], ctypes.c_int),
             "fsg_server_footprint": ([vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
             "fsg_server_clock": ([vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
-            "fsg_server_submit": ([vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
+            "fsg_server_submit": ([vp, vp, vp, i64, i64, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
                                   ctypes.c_int),
             "fsg_server_wait": ([vp, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)], ctypes.c_int),
             "fsg_postproc_device": ([ctypes.POINTER(PostDesc), vp, i64, i64, vp, vp, vp, vp, vp, vp, vp],
                                     ctypes.c_int),
-            "fsg_server_submit_layers": ([vp, vp, vp, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp, vp,
+            "fsg_server_submit_layers": ([vp, vp, vp, i64, i64, ctypes.POINTER(PostDesc), vp, vp, vp, vp, vp,
                                           ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "fsg_host_alloc": ([ctypes.c_size_t], vp),
             "fsg_host_free": ([vp], ctypes.c_int),
@@ -203,7 +203,7 @@ def load(path=None):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        if lib.fsg_abi_version() != 5:
+        if lib.fsg_abi_version() != 6:
             raise FsgError("libfsgpu.so ABI version mismatch")
         if path is None:
             _lib = lib

@@ -430,8 +430,9 @@ int ctx_nz(const fsg_ctx *c) { return n_fields(c); }
 size_t ctx_z_bytes(const fsg_ctx *c, int64_t n) { return z_bytes(c, n); }
 // pixels [0, n) of pix into columns [0, n) of outputs with row stride ld
 int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, double *Z, int32_t *U,
-                     int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks) {
-    return launch(c, pix, n, Z, U, sr, si, work, stream, false, ld, max_blocks);
+                     int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks,
+                     int64_t grid_w) {
+    return launch(c, pix, n, Z, U, sr, si, work, stream, false, ld, max_blocks, grid_w);
 }
 size_t work_words() { return WORK_WORDS; }
 }  // namespace fsg

@@ -47,7 +47,8 @@ int ctx_holomorphic(const fsg_ctx *c);
 int ctx_nz(const fsg_ctx *c);
 size_t ctx_z_bytes(const fsg_ctx *c, int64_t n);
 int ctx_launch_range(fsg_ctx *c, const double *pix, int64_t n, int64_t ld, double *Z, int32_t *U,
-                     int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks);
+                     int8_t *sr, int32_t *si, unsigned long long *work, hipStream_t stream, int max_blocks,
+                     int64_t grid_w);
 size_t work_words();
 int post_launch(const fsg_post_desc *pp, const double *Z, int64_t n, int64_t ld, const double *pix,
                 const int32_t *si, const int8_t *sr, float *nu, float *dem, float *nrm, hipStream_t stream);
@@ -128,6 +129,7 @@ struct Req {
     fsg_post_desc pp{};
     float *nu_h = nullptr, *dem_h = nullptr, *nrm_h = nullptr;
     int64_t n = 0;
+    int64_t grid_w = 0;       // > 0: the pixels are rows of grid_w, each chunk taken in 8x8 blocks
     Layout lay{};
     int64_t off = -1;
     char *dev = nullptr;
@@ -318,7 +320,7 @@ void fsg_server::loop() {
                                       reinterpret_cast<int32_t *>(d + best->lay.u) + start,
                                       reinterpret_cast<int8_t *>(d + best->lay.sr) + start,
                                       reinterpret_cast<int32_t *>(d + best->lay.si) + start,
-                                      ln.work, ln.s, -(int)lanes.size());
+                                      ln.work, ln.s, -(int)lanes.size(), best->grid_w);
             if (rc != 0) fail_req(best, rc, fsg_last_error());
             (void)hipEventRecord(ln.ev, ln.s);
             ln.req = best;
@@ -423,31 +425,33 @@ static int submit_req(fsg_server *s, Req *r, uint64_t *ticket);
 
 extern "C" {
 
-int fsg_server_submit(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z, int32_t *U,
-                      int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket) {
+int fsg_server_submit(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w, double *Z,
+                      int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket) {
     if (!s || !ctx || !ticket) return set_error(FSG_E_INVALID, "null argument");
     if (n < 0 || (n > 0 && (!c_pix || !Z || !U || !stop_reason || !stop_iter)))
         return set_error(FSG_E_INVALID, "bad tile buffers");
+    if (grid_w < 0) return set_error(FSG_E_INVALID, "grid_w must be >= 0");
     Req *r = new (std::nothrow) Req();
     if (!r) return set_error(FSG_E_NOMEM, "out of host memory");
-    r->ctx = ctx; r->pix_h = c_pix; r->n = n;
+    r->ctx = ctx; r->pix_h = c_pix; r->n = n; r->grid_w = grid_w;
     r->Z_h = Z; r->U_h = U; r->sr_h = stop_reason; r->si_h = stop_iter;
     r->lay = layout_of(ctx, n);
     return submit_req(s, r, ticket);
 }
 
-int fsg_server_submit_layers(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t n,
+int fsg_server_submit_layers(fsg_server *s, fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w,
                              const fsg_post_desc *pp, float *nu, float *dem, float *normal,
                              int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket) {
     if (!s || !ctx || !ticket || !pp) return set_error(FSG_E_INVALID, "null argument");
     if (n < 0 || (n > 0 && (!c_pix || !stop_reason || !stop_iter)))
         return set_error(FSG_E_INVALID, "bad tile buffers");
+    if (grid_w < 0) return set_error(FSG_E_INVALID, "grid_w must be >= 0");
     if (!(pp->d > 1.) || !(pp->M > 0.) || ((dem || normal) && pp->dzndc_row < 1) ||
         pp->dzndc_row >= ctx_nz(ctx) || pp->holomorphic != ctx_holomorphic(ctx))
         return set_error(FSG_E_INVALID, "post-processing descriptor does not match the context");
     Req *r = new (std::nothrow) Req();
     if (!r) return set_error(FSG_E_NOMEM, "out of host memory");
-    r->ctx = ctx; r->pix_h = c_pix; r->n = n;
+    r->ctx = ctx; r->pix_h = c_pix; r->n = n; r->grid_w = grid_w;
     r->sr_h = stop_reason; r->si_h = stop_iter;
     r->layers = true; r->pp = *pp; r->nu_h = nu; r->dem_h = dem; r->nrm_h = normal;
     r->lay = layout_of(ctx, n, true);

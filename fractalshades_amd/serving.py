@@ -115,29 +115,31 @@ class TileServer:
         check(load().fsg_server_footprint(ctx.handle, n, ctypes.byref(b)))
         return b.value
 
-    def submit(self, ctx, tile):
+    def submit(self, ctx, tile, grid_w=0):
         """Queue tile (a PinnedTile or any object with the five C-contiguous
-        arrays) on context ctx (a GpuContext); returns the ticket."""
+        arrays) on context ctx (a GpuContext); returns the ticket.  grid_w >
+        0: the pixels are rows of grid_w, iterated in 8x8 blocks (the same
+        results)."""
         t = ctypes.c_uint64()
         arrs = (tile.c_pix, tile.Z, tile.U, tile.stop_reason, tile.stop_iter)
         for a in arrs:
             if not a.flags.c_contiguous:
                 raise ValueError("request buffers must be C-contiguous")
         check(load().fsg_server_submit(self._h, ctx.handle, tile.c_pix.ctypes.data, tile.c_pix.shape[0],
-                                       tile.Z.ctypes.data, tile.U.ctypes.data, tile.stop_reason.ctypes.data,
+                                       int(grid_w), tile.Z.ctypes.data, tile.U.ctypes.data, tile.stop_reason.ctypes.data,
                                        tile.stop_iter.ctypes.data, ctypes.byref(t)))
         with self._lock:
             self._keep[t.value] = (ctx, tile)   # alive until waited for
         return t.value
 
-    def submit_layers(self, ctx, tile, post):
+    def submit_layers(self, ctx, tile, post, grid_w=0):
         """Queue a request returning the nu / dem (/ normal) layers
         (LayerTile) instead of the raw fields; post is a PostDesc
-        (postproc.post_desc)."""
+        (postproc.post_desc); grid_w as for submit."""
         t = ctypes.c_uint64()
         nrm = tile.normal.ctypes.data if tile.normal is not None else None
         check(load().fsg_server_submit_layers(
-            self._h, ctx.handle, tile.c_pix.ctypes.data, tile.c_pix.shape[0], ctypes.byref(post),
+            self._h, ctx.handle, tile.c_pix.ctypes.data, tile.c_pix.shape[0], int(grid_w), ctypes.byref(post),
             tile.nu.ctypes.data, tile.dem.ctypes.data, nrm, tile.stop_reason.ctypes.data,
             tile.stop_iter.ctypes.data, ctypes.byref(t)))
         with self._lock:
@@ -400,12 +402,14 @@ def layer_tiles(reqs):
     return tiles
 
 
-def run_stream(server, reqs, rate_hz=None, seed=1, tiles=None, posts=None):
+def run_stream(server, reqs, rate_hz=None, seed=1, tiles=None, posts=None, widths=None):
     """Submit prepared requests [(cia, c_pix)] with Poisson arrivals at
     `rate_hz` (all at once when None); wait for all.  With `posts` (one
     PostDesc per request) the requests return the nu / dem layers instead of
-    the raw fields.  Returns per-request dicts: latency_ms (arrival ->
-    results in host memory), queue_ms, pixels, pixel_iters, tile."""
+    the raw fields; with `widths` (one row width per request) each request's
+    pixels are iterated in 8x8 blocks of its rows.  Returns per-request
+    dicts: latency_ms (arrival -> results in host memory), queue_ms, pixels,
+    pixel_iters, tile."""
     rng = np.random.default_rng(seed)
     if tiles is None:
         tiles = layer_tiles(reqs) if posts is not None else pinned_tiles(reqs)
@@ -418,10 +422,11 @@ def run_stream(server, reqs, rate_hz=None, seed=1, tiles=None, posts=None):
         delay = ta - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
+        gw = widths[k] if widths is not None else 0
         if posts is not None:
-            tickets.append(server.submit_layers(cia.ctx, tile, posts[k]))
+            tickets.append(server.submit_layers(cia.ctx, tile, posts[k], grid_w=gw))
         else:
-            tickets.append(server.submit(cia.ctx, tile))
+            tickets.append(server.submit(cia.ctx, tile, grid_w=gw))
     out = []
     for (cia, _), tile, tk, ta in zip(reqs, tiles, tickets, arrivals):
         info = server.wait(tk)

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 5
+#define FSG_ABI_VERSION 6
 
 /* model kinds */
 #define FSG_KIND_M2 0   /* Perturbation_mandelbrot, models/mandelbrot_M2.py:359-627 */
@@ -319,13 +319,16 @@ int fsg_server_footprint(const fsg_ctx *ctx, int64_t n, int64_t *bytes);
 /* Milliseconds since the server started. */
 int fsg_server_clock(fsg_server *srv, double *ms);
 /* Queue a request (same buffers as fsg_tile); returns at once with a ticket.
+ * grid_w > 0: the pixels are rows of grid_w (as for fsg_tile_grid) and each
+ * chunk is iterated in 8x8 blocks of its rows -- the same results at the
+ * same indices, with less divergence; 0: index order.
  * FSG_E_NOMEM if the request alone exceeds the budget. */
-int fsg_server_submit(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, double *Z,
-                      int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
+int fsg_server_submit(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w,
+                      double *Z, int32_t *U, int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
 /* The same request returning the post-processing layers (nu, dem: float32
  * [n]; normal: float32 [n][2]; any may be null) with stop_reason and
  * stop_iter instead of the raw fields: the raw results stay on the device. */
-int fsg_server_submit_layers(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n,
+int fsg_server_submit_layers(fsg_server *srv, fsg_ctx *ctx, const double *c_pix, int64_t n, int64_t grid_w,
                              const fsg_post_desc *pp, float *nu, float *dem, float *normal,
                              int8_t *stop_reason, int32_t *stop_iter, uint64_t *ticket);
 /* Wait for a request (timeout_ms < 0: no limit).  Returns its status, or

@@ -18,26 +18,26 @@ def with_fields(args, **over):
 
 
 def _mix():
-    """(label, CycleIndepArgs, c_pix) for a mix of models, sizes, depths and
-    precisions."""
+    """(label, CycleIndepArgs, c_pix, row width) for a mix of models, sizes,
+    depths and precisions."""
     out = []
     f = frames.make("M2_E20", nx=96)
     a, c = frames.frame_inputs(f)
-    out += [("M2_E20 fp64", a, c), ("M2_E20 xr", with_fields(a, xr_mode=1), c),
-            ("M2_E20 fp32x", with_fields(a, precision=1), c)]
+    out += [("M2_E20 fp64", a, c, 96), ("M2_E20 xr", with_fields(a, xr_mode=1), c, 96),
+            ("M2_E20 fp32x", with_fields(a, precision=1), c, 96)]
     f = frames.make("M2_divref5", nx=64)
     a, c = frames.frame_inputs(f)
-    out += [("M2_divref5", a, c)]
+    out += [("M2_divref5", a, c, 64)]
     f = frames.make("M2_E11", nx=40, interior_detect=True)
     a, c = frames.frame_inputs(f)
-    out += [("M2_E11 interior xr", with_fields(a, xr_mode=1), c)]
+    out += [("M2_E11 interior xr", with_fields(a, xr_mode=1), c, 40)]
     f = frames.mn_frame(3, nx=48)
     a, c = frames.frame_inputs(f)
-    out += [("Mn3", a, c)]
+    out += [("Mn3", a, c, 48)]
     f = frames.make("BS_ex14", nx=56, max_iter=4000)
     a, c = frames.frame_inputs(f)
-    out += [("BS_ex14", a, c), ("BS_ex14 xr", with_fields(a, xr_mode=1), c),
-            ("BS_ex14 fp32x", with_fields(a, precision=1), c)]
+    out += [("BS_ex14", a, c, 56), ("BS_ex14 xr", with_fields(a, xr_mode=1), c, 56),
+            ("BS_ex14 fp32x", with_fields(a, precision=1), c, 56)]
     return out
 
 
@@ -51,17 +51,19 @@ def _tile_of(args, c_pix):
 def test_server_mix_equals_alone():
     mix = _mix()
     alone = []
-    for label, a, c in mix:
+    for label, a, c, _ in mix:
         print(f"alone {label} n={c.size}", flush=True)
         alone.append(run_gpu(a, c)[0])
     srv = TileServer(lanes=3, chunk_px=1024, hbm_budget=256 << 20)
     try:
         tiles, tickets = [], []
-        for rep in range(2):            # every request twice, interleaved
-            for label, a, c in mix:
+        # every request twice, interleaved: in index order, then in 8x8
+        # blocks of its rows (chunks of 1024 pixels start inside rows)
+        for rep in range(2):
+            for label, a, c, w in mix:
                 t = _tile_of(a, c)
                 tiles.append((label, t))
-                tickets.append(srv.submit(a.ctx, t))
+                tickets.append(srv.submit(a.ctx, t, grid_w=w if rep else 0))
         infos = [srv.wait(tk, 120000) for tk in tickets]
     finally:
         srv.close()

@@ -23,7 +23,7 @@ def test_library_exports_all_declared():
     lib = native.load()
     for s in declared_symbols():
         assert hasattr(lib, s), s
-    assert lib.fsg_abi_version() == 5
+    assert lib.fsg_abi_version() == 6
     assert lib.fsg_work_bytes() >= 40
 
 
