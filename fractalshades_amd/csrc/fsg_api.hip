@@ -42,7 +42,6 @@ __global__ void k_fill_i8(int8_t *p, int64_t n, int8_t v) {
     if (i < n) p[i] = v;
 }
 
-constexpr int WORK_WORDS = 8;
 
 template <class T> T *dalloc(size_t n, hipError_t &err) {
     void *p = nullptr;
@@ -399,7 +398,7 @@ int launch(fsg_ctx *c, const double *pix, int64_t n, double *Z, int32_t *U, int8
     fill_args(c, a);
     a.pix = pix; a.npts = n; a.ld = ld < 0 ? n : ld; a.Z = Z; a.U = U; a.stop_reason = sr; a.stop_iter = si;
     a.work = work; a.max_blocks = max_blocks; a.grid_w = grid_w; a.stop = stop;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, work, WORK_WORDS);
+    hipLaunchKernelGGL(k_zero_u64, dim3((WORK_WORDS + 255) / 256), dim3(256), 0, stream, work, WORK_WORDS);
     FSG_HIP(hipGetLastError());
     if (n == 0) return 0;
     if (timed) FSG_HIP(hipEventRecord(c->ev0, stream));
@@ -507,7 +506,8 @@ int tile_staged(fsg_ctx *c, const double *c_pix, int64_t n, int64_t grid_w, doub
 // copy for the life of the process, and no call allocates.  Its address on a
 // device comes from hipGetSymbolAddress, taken once per device with that
 // device current; host code uses that address, never the array's name.
-constexpr size_t NOALLOC_WORK_BYTES = 256;       // cursor and counters, then the tile staging
+constexpr size_t NOALLOC_WORK_BYTES = 4096;      // cursors and counters, then the tile staging
+static_assert(NOALLOC_WORK_BYTES >= WORK_WORDS * sizeof(unsigned long long), "workspace does not fit");
 static __device__ __align__(256) unsigned char scratch[FSG_NOALLOC_SCRATCH_BYTES];
 
 struct ScratchDev {

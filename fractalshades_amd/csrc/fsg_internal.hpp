@@ -9,6 +9,13 @@ enum Kind { KIND_M2 = 0, KIND_MN = 1, KIND_BS = 2 };
 // Tables::zn_k codes below this one name an Xrange register slot
 constexpr int32_t ZK_REGISTER = -(1 << 28);
 
+// Workspace of a tile launch (KArgs::work), 64-bit words: [1..5] step
+// counters, [6] unsure pixels, then the pixel cursors, one per XCD, each in
+// a 128-byte line of its own: word WORK_CURSORS + 16 x for the first pass,
+// WORK_CURSORS + 128 + 16 x for the binary64 pass over the unsure pixels.
+constexpr int WORK_CURSORS = 16;
+constexpr int WORK_WORDS = WORK_CURSORS + 2 * 8 * 16;
+
 // Read-only per-reference data, resident in HBM for the life of a context.
 struct Tables {
     const cplx *Zn;            // [L] reference orbit Z_n rounded to binary64
@@ -87,9 +94,9 @@ struct KArgs {
     int32_t calc_orbit, backshift;    // orbit point at exit - backshift in the last field(s)
     LdsPlan lds;                      // lds.bytes == 0: tables read from global memory
     LdsPlanX ldsx;                    // the same for the binary32 form
-    // workspace: [0] pixel cursor, [1..5] step counters (plain, bla, bla iters,
-    // rebase, bla lookups), [6] pixels the binary32 form left unsure, [7] the
-    // cursor of the binary64 pass over them
+    // workspace (WORK_WORDS words): [1..5] step counters (plain, bla, bla
+    // iters, rebase, bla lookups), [6] pixels the binary32 form left unsure,
+    // the pixel cursors from WORK_CURSORS on
     unsigned long long *work;
     // 1: the binary64 pass over the pixels whose stop_reason the binary32
     // pass set to REASON_UNSURE (every other pixel is skipped)

@@ -3,7 +3,7 @@
 // k_tile: persistent, work-stealing tile kernel.  Each lane owns one pixel at
 // a time and runs the reference's per-pixel loop (BLA jump, perturbation step,
 // max_iter / interior / escape tests, rebasing); when its pixel stops it
-// writes the results and takes the next pixel index from a global cursor.
+// writes the results and takes the next pixel index from the pixel cursors.
 // Refill is wave-aggregated: one atomic per wave hands out consecutive indices
 // to all lanes that need work (ballot + mbcnt).  The arithmetic is scalar
 // complex recurrences (no MFMA); the kernel is bound by the FP64 VALU rate.
