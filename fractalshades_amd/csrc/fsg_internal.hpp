@@ -15,6 +15,9 @@ constexpr int32_t ZK_REGISTER = -(1 << 28);
 // WORK_CURSORS + 128 + 16 x for the binary64 pass over the unsure pixels.
 constexpr int WORK_CURSORS = 16;
 constexpr int WORK_WORDS = WORK_CURSORS + 2 * 8 * 16;
+// LDS the tile loop declares itself (its counters' block sums), kept out of
+// the budget for staged tables
+constexpr int TILE_STATIC_LDS = 1024;
 
 // Read-only per-reference data, resident in HBM for the life of a context.
 struct Tables {

@@ -233,6 +233,7 @@ int max_lds_bytes() {
     if (cached < 0) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 0;
+        v = std::max(0, v - TILE_STATIC_LDS);     // the tile loop's own LDS
         per_dev[dev].store(v + 1, std::memory_order_relaxed);
         cached = v;
     }
