@@ -27,6 +27,16 @@ static tile_fn fn_lds_m2xr(bool full, int der) {
     }
 }
 
+// the same with binary64 deltas
+static tile_fn fn_lds_m2std(bool full, int der) {
+    switch (der) {
+        case 0: return fn_lds<HoloStd<KIND_M2, 2, false, 0>>(full);
+        case 1: return fn_lds<HoloStd<KIND_M2, 2, false, 1>>(full);
+        case 2: return fn_lds<HoloStd<KIND_M2, 2, false, 2>>(full);
+        default: return fn_lds<HoloStd<KIND_M2, 2, false, 3>>(full);
+    }
+}
+
 tile_fn select_tile_lds(const LaunchCfg &c, bool full, int der) {
     if (tile_lds_block(c) == 512) {
         switch (c.degree) {
@@ -36,7 +46,7 @@ tile_fn select_tile_lds(const LaunchCfg &c, bool full, int der) {
             default: return nullptr;
         }
     }
-    if (c.kind == KIND_M2) return c.xr ? fn_lds_m2xr(full, der) : fn_lds<HoloStd<KIND_M2, 2>>(full);
+    if (c.kind == KIND_M2) return c.xr ? fn_lds_m2xr(full, der) : fn_lds_m2std(full, der);
     if (c.kind == KIND_MN) {
         switch (c.degree) {
 #define FSG_MN(D) case D: return c.xr ? fn_lds<HoloXr<KIND_MN, D>>(full) : fn_lds<HoloStd<KIND_MN, D>>(full);

@@ -133,7 +133,14 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
     bool exhausted = false;   // every cursor is drained for this lane
     for (;;) {
         const unsigned long long act = __ballot(active);
-        if (act == 0ull || 64 - __popcll(act) >= REFILL_BATCH) {
+        // idle lanes with something to do at a refill: a stopped pixel to
+        // write, or cursors not all found drained.  Once there are none, the
+        // wave only steps its remaining pixels (a launch's last, longest
+        // ones) without the refill's ballots, and leaves the loop when none
+        // is left: every result is then written.
+        const unsigned long long want = __ballot(!active && (pending || !exhausted));
+        if (act == 0ull && want == 0ull) break;
+        if (want != 0ull && (act == 0ull || 64 - __popcll(act) >= REFILL_BATCH)) {
             bool drained = false;     // this lane found the current cursor drained
             if (!active) {
                 if (pending) {
@@ -177,11 +184,9 @@ __device__ __forceinline__ void tile_loop(const KArgs &a) {
                 }
             }
             if (__ballot(drained) != 0ull) cur++;
-            // every lane idle after the refill: done once every lane has
-            // found the cursor exhausted (every result is then written); the
-            // binary64 pass can draw a whole batch of slots without an unsure
-            // pixel, and then refills again
-            if (__ballot(active) == 0ull && __ballot(!exhausted) == 0ull) break;
+            // lanes still idle after the refill (the binary64 pass can draw a
+            // whole batch of slots without an unsure pixel) refill again on
+            // the next trip
         }
         if (active) {
             reason = Model::step(s, a, k);

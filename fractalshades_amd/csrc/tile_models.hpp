@@ -148,9 +148,13 @@ __device__ __forceinline__ xl c_from_pix_xl(const KArgs &a, double px, double py
 }
 
 // --------------------------------------------------------------------------
-// Holomorphic models, binary64 deltas
-template <int KIND, int DEG, bool ORB = false>
+// Holomorphic models, binary64 deltas.  DER >= 0 fixes the derivatives at
+// compile time (bit 0: dz/dc, bit 1: dz/dz); DER = -1 reads them from the
+// launch arguments.
+template <int KIND, int DEG, bool ORB = false, int DER = -1>
 struct HoloStd {
+    __device__ static bool DC_ON(const KArgs &a) { if constexpr (DER >= 0) return (DER & 1) != 0; else return a.calc_dc != 0; }
+    __device__ static bool DZ_ON(const KArgs &a) { if constexpr (DER >= 0) return (DER & 2) != 0; else return a.calc_dz != 0; }
     struct State { cplx z, d, h, c; int32_t w, n; bool dyn, nul; OrbitRec<ORB> orb; };
 
     __device__ static void init(State &s, const KArgs &a, double px, double py) {
@@ -173,44 +177,68 @@ struct HoloStd {
                 s.n += st;
                 s.w += st;                      // w + st < first_invalid <= order
                 s.z = A * s.z + B * s.c;
-                if (a.calc_dc) s.d = A * s.d;
-                if (a.calc_dz) s.h = A * s.h;
+                if (DC_ON(a)) s.d = A * s.d;
+                if (DZ_ON(a)) s.h = A * s.h;
                 k.bla++; k.bla_it += st;
             }
         }
         s.n += 1; k.plain++;
+        // Every table value the step reads is loaded here, before any of it
+        // is used: the orbit point and path values at w, the next orbit point
+        // (escape and rebase tests) and the dZndz value of the interior test,
+        // whose indexes are known at the start of the step.  A pixel whose
+        // wave has no other work left (the last, longest pixels of a launch)
+        // then waits for one table round trip per step instead of one per
+        // test.
+        const int32_t w1 = (s.w + 1 >= t.order) ? 0 : s.w + 1;
         const cplx Zr = t.Zn[s.w];
+        const cplx Z1 = t.Zn[w1];
+        cplx Zd = zc(), Zh = zc(), H1 = zc();
+        if (DC_ON(a)) { const cplx v = t.dc[s.w]; Zd = s.dyn ? zc() : v; }
+        if (DZ_ON(a)) {
+            Zh = t.dz[s.nul ? 0 : s.w];
+            H1 = t.dz[s.nul ? 0 : (s.n == t.order ? t.L : w1)];
+        }
         if constexpr (KIND == KIND_MN) {
-            const cplx Zd = a.calc_dc ? (s.dyn ? zc() : t.dc[s.w]) : zc();
-            const cplx Zh = a.calc_dz ? (s.nul ? t.dz[0] : t.dz[s.w]) : zc();
-            s.z = mn_step<DEG>(s.z, Zr, s.c, a.calc_dc, a.calc_dz, s.d, s.h, s.d, s.h, Zd, Zh,
+            s.z = mn_step<DEG>(s.z, Zr, s.c, DC_ON(a), DZ_ON(a), s.d, s.h, s.d, s.h, Zd, Zh,
                                [](cplx u, cplx v) { return u + v; });
         } else {
-            if (a.calc_dc) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, s.dyn ? zc() : t.dc[s.w]);
-            if (a.calc_dz) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, s.nul ? t.dz[0] : t.dz[s.w]);
+            if (DC_ON(a)) s.d = p_deriv<KIND, DEG>(s.z, s.d, Zr, Zd);
+            if (DZ_ON(a)) s.h = p_deriv<KIND, DEG>(s.z, s.h, Zr, Zh);
             s.z = p_zn<KIND, DEG>(s.z, Zr, s.c);
         }
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        if (s.n >= a.max_iter) return 0;
-        if (a.calc_dz) {
-            cplx nx = s.nul ? t.dz[0] : (s.n == t.order ? t.dz[t.L] : t.dz[s.w]);
-            cplx q = s.h + nx;
-            if (q.re * q.re + q.im * q.im < a.eps2) return 2;
+        s.w = w1;
+        // The stop and rebase tests are evaluated together: a step that meets
+        // none of them (nearly every step) leaves by one branch, and the
+        // tests' own order (max_iter, interior, escape, rebase) is kept on
+        // the other path.  (The orbit recorder runs between the tests: its
+        // kernels take that path on every step.)
+        bool t_int = false;
+        if (DZ_ON(a)) {
+            const cplx q = s.h + H1;
+            t_int = q.re * q.re + q.im * q.im < a.eps2;
         }
-        const cplx ZZ = s.z + t.Zn[s.w];
+        const cplx ZZ = s.z + Z1;
+        const bool t_esc = ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2;
+        const bool reb_div = s.w >= t.div_iter - 1;
+        const bool dyn = (fabs(ZZ.re) <= fabs(s.z.re)) && (fabs(ZZ.im) <= fabs(s.z.im));
+        if constexpr (!ORB) {
+            if (!(s.n >= a.max_iter || t_int || t_esc || reb_div || dyn)) { s.dyn = false; return -1; }
+        }
+        if (s.n >= a.max_iter) return 0;
+        if (t_int) return 2;
         s.orb.record(s.n, a.backshift, ZZ.re, ZZ.im);
-        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
-        bool reb = (s.w >= t.div_iter - 1);
+        if (t_esc) return 1;
+        bool reb = reb_div;
         if (!reb) {
-            s.dyn = (fabs(ZZ.re) <= fabs(s.z.re)) && (fabs(ZZ.im) <= fabs(s.z.im));
-            reb = s.dyn;
+            s.dyn = dyn;
+            reb = dyn;
         }
         if (reb) {
             s.z = ZZ;
-            if (a.calc_dc) s.d = s.d + t.dc[s.w];
-            if (a.calc_dz) {
-                if (!s.nul) s.h = s.h + (s.n == t.order ? t.dz[t.L] : t.dz[s.w]);
+            if (DC_ON(a)) s.d = s.d + t.dc[s.w];
+            if (DZ_ON(a)) {
+                if (!s.nul) s.h = s.h + H1;     // dZndz at w (at L after a full period)
                 s.nul = true;
             }
             s.w = 0;
@@ -224,8 +252,8 @@ struct HoloStd {
         cplx z = s.z + a.t.Zn[s.w];
         a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
         int f = 1;
-        if (a.calc_dz) { a.Z[2 * (n + i)] = s.h.re; a.Z[2 * (n + i) + 1] = s.h.im; f = 2; }
-        if (a.calc_dc) {
+        if (DZ_ON(a)) { a.Z[2 * (n + i)] = s.h.re; a.Z[2 * (n + i) + 1] = s.h.im; f = 2; }
+        if (DC_ON(a)) {
             cplx d = s.d + a.t.dc[s.w];
             a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
         }
@@ -252,17 +280,15 @@ struct HoloXr {
     // |mantissa| within 2^+-100
     __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xz_ld(m.re, m.im, t.dc_e[i]); }
     __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xz_ld(m.re, m.im, t.dz_e[i]); }
-    // orbit point i, normalised: the register's Xrange value where the
-    // binary64 orbit underflows, the binary64 value elsewhere, scaled by the
-    // exponent the tables hold for it (xl_of of the point, without its
+    // an orbit point, normalised, from its code k and binary64 value z: the
+    // register's Xrange value where the binary64 orbit underflows, z
+    // elsewhere, scaled by the exponent k (xl_of of the point, without its
     // exponent extraction)
-    __device__ static xl ZR(const Tables &t, int32_t i) {
-        const int32_t k = t.zn_k[i];
+    __device__ static xl ZRv(const Tables &t, int32_t k, cplx z) {
         if (k < ZK_REGISTER) {
             const int32_t j = k - INT32_MIN;
             return xl_of(t.xr_m[2 * j], t.xr_m[2 * j + 1], t.xr_e[j]);
         }
-        const cplx z = t.Zn[i];
         xl r;
         r.re = ldexp(z.re, -k); r.im = ldexp(z.im, -k); r.e = k;
         return r;
@@ -296,42 +322,56 @@ struct HoloXr {
             }
         }
         s.n += 1; k.plain++;
-        const xz Zr = as_z(ZR(t, s.w));
+        // Every table value the step reads is loaded here, as in HoloStd: the
+        // orbit point, its code and the path values at w, the next orbit
+        // point and its code (escape and rebase tests) and the dZndz value of
+        // the interior test
+        const int32_t w1 = (s.w + 1 >= t.order) ? 0 : s.w + 1;
+        const int32_t k0 = t.zn_k[s.w], k1 = t.zn_k[w1];
+        const cplx Z0 = t.Zn[s.w], Z1 = t.Zn[w1];
+        xz Zd = xz_zero(), Zh = xz_zero(), H1 = xz_zero();
+        if (DC_ON(a)) { const xz v = DCz(t, s.w); Zd = s.dyn ? xz_zero() : v; }
+        if (DZ_ON(a)) {
+            Zh = DZz(t, s.nul ? 0 : s.w);
+            H1 = DZz(t, s.nul ? 0 : (s.n == t.order ? t.L : w1));
+        }
+        const xz Zr = as_z(ZRv(t, k0, Z0));
         const xz z0 = as_z(s.z);
         if constexpr (KIND == KIND_M2) {
             // mandelbrot_M2.py:607-622 with Z + z formed once for both
             // derivatives and the factors 2 taken into the exponents (exact)
             const xz Zz = Zr + z0;
-            if (DC_ON(a)) s.d = add_n(Zz * as_z(s.d), (s.dyn ? xz_zero() : DCz(t, s.w)) * z0, 1);
-            if (DZ_ON(a)) s.h = add_n(Zz * as_z(s.h), (s.nul ? DZz(t, 0) : DZz(t, s.w)) * z0, 1);
+            if (DC_ON(a)) s.d = add_n(Zz * as_z(s.d), Zd * z0, 1);
+            if (DZ_ON(a)) s.h = add_n(Zz * as_z(s.h), Zh * z0, 1);
             s.z = add_n(z0 * (z0 + muls(2., Zr)), as_z(s.c));
         } else {
-            const xz Zd = DC_ON(a) ? (s.dyn ? xz_zero() : DCz(t, s.w)) : xz_zero();
-            const xz Zh = DZ_ON(a) ? (s.nul ? DZz(t, 0) : DZz(t, s.w)) : xz_zero();
             // the last sums normalised at once (add_n: no zero test of their own)
             s.z = mn_step<DEG>(z0, Zr, as_z(s.c), DC_ON(a), DZ_ON(a), s.d, s.h, as_z(s.d), as_z(s.h),
                                Zd, Zh, [](xz u, xz v) { return add_n(u, v); });
         }
         s.zs = to_std(s.z);
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        if (s.n >= a.max_iter) return 0;
-        if (DZ_ON(a)) {
-            const xz nx = s.nul ? DZz(t, 0) : (s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
-            if (abs2_lt(add_raw(as_z(s.h), nx), a.eps2)) return 2;
+        s.w = w1;
+        // the stop and rebase tests evaluated together, as in HoloStd
+        const bool t_int = DZ_ON(a) && abs2_lt(add_raw(as_z(s.h), H1), a.eps2);
+        const cplx ZZ = s.zs + Z1;
+        const bool t_esc = ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2;
+        const bool reb_div = s.w >= t.div_iter - 1;
+        const bool dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
+        if constexpr (!ORB) {
+            if (!(s.n >= a.max_iter || t_int || t_esc || reb_div || dyn)) { s.dyn = false; return -1; }
         }
-        const cplx ZZ = s.zs + t.Zn[s.w];
+        if (s.n >= a.max_iter) return 0;
+        if (t_int) return 2;
         s.orb.record(s.n, a.backshift, ZZ.re, ZZ.im);
-        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
+        if (t_esc) return 1;
         // both rebases (perturbation.py:1283-1372) share the derivative
         // updates: one block, whichever of them a lane takes
-        const bool reb_div = s.w >= t.div_iter - 1;
         bool reb = reb_div;
         xl ZZx;
         if (!reb_div) {
-            s.dyn = (fabs(ZZ.re) <= fabs(s.zs.re)) && (fabs(ZZ.im) <= fabs(s.zs.im));
+            s.dyn = dyn;
             if (s.dyn) {
-                ZZx = s.z + ZR(t, s.w);
+                ZZx = s.z + ZRv(t, k1, Z1);
                 reb = le_abs2(ZZx, s.z);
             }
         }
@@ -341,7 +381,7 @@ struct HoloXr {
             // the path values unnormalised: the sum's normalisation covers them
             if (DC_ON(a)) s.d = add_n(as_z(s.d), DCz(t, s.w));
             if (DZ_ON(a)) {
-                if (!s.nul) s.h = add_n(as_z(s.h), s.n == t.order ? DZz(t, t.L) : DZz(t, s.w));
+                if (!s.nul) s.h = add_n(as_z(s.h), H1);   // dZndz at w (at L after a full period)
                 s.nul = true;
             }
             s.w = 0;

@@ -4,7 +4,8 @@ XCD's cursor and then from the next ones in turn.  Checked on the CPU, with
 waves interleaved at random and pixels of random length: every slot is
 handed out exactly once, the binary64 second pass takes exactly the unsure
 pixels, and every wave ends (a wave ends once it has found all eight cursors
-drained) -- for empty, tiny and ragged launches and grids that do not fill
+drained and written every stopped pixel), and every pixel's results are
+written exactly once -- for empty, tiny and ragged launches and grids that do not fill
 the eight XCDs."""
 import random
 
@@ -20,8 +21,9 @@ def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=40):
     rnd = random.Random(seed)
     cursors = [0] * 8
     gsh = group_shift(npts)
-    started, skipped = [], []
-    waves = [dict(xcd=b & 7, cur=0, active=[False] * 64, left=[0] * 64, exhausted=[False] * 64)
+    started, skipped, written = [], [], []
+    waves = [dict(xcd=b & 7, cur=0, active=[False] * 64, left=[0] * 64, exhausted=[False] * 64,
+                  pending=[None] * 64)
              for b in range(nblocks) for _ in range(waves_per_block)]
     live = list(range(len(waves)))
     trips = 0
@@ -31,8 +33,17 @@ def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=40):
         trips += 1
         assert trips < 5_000_000, "a wave does not end"
         act = sum(w["active"])
-        if act == 0 or 64 - act >= refill_batch:
+        want = [l for l in range(64) if not w["active"][l]
+                and (w["pending"][l] is not None or not w["exhausted"][l])]
+        if act == 0 and not want:
+            live.pop(k)
+            continue
+        if want and (act == 0 or 64 - act >= refill_batch):
             drained_any = False
+            for lane in range(64):
+                if not w["active"][lane] and w["pending"][lane] is not None:
+                    written.append(w["pending"][lane])
+                    w["pending"][lane] = None
             idle = [l for l in range(64) if not w["active"][l] and not w["exhausted"][l]]
             if idle:
                 cx = (w["xcd"] + w["cur"]) & 7
@@ -46,6 +57,8 @@ def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=40):
                             started.append(idx)
                             w["active"][lane] = True
                             w["left"][lane] = rnd.randint(1, 40)
+                            w["pixel"] = w.get("pixel", {})
+                            w["pixel"][lane] = idx
                         else:
                             skipped.append(idx)
                     else:
@@ -53,13 +66,13 @@ def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=40):
                         w["exhausted"][lane] = w["cur"] >= 7
             if drained_any:
                 w["cur"] += 1
-            if not any(w["active"]) and all(w["exhausted"]):
-                live.pop(k)
-                continue
         for lane in range(64):
             if w["active"][lane]:
                 w["left"][lane] -= 1
-                w["active"][lane] = w["left"][lane] > 0
+                if w["left"][lane] == 0:
+                    w["active"][lane] = False
+                    w["pending"][lane] = w["pixel"][lane]
+    assert sorted(written) == sorted(started), "a stopped pixel was not written exactly once"
     return started, skipped
 
 
