@@ -150,141 +150,6 @@ hipError_t build_xs_tables(const Tables &t, int64_t bla_len, TablesX &x, hipStre
     return hipGetLastError();
 }
 
-// BLA stage selection of ref_BLA_get (perturbation.py:2116-2170), on |z|
-__device__ __forceinline__ int32_t bla_lookup_az(const Tables &t, double az, int32_t w, int32_t &idx) {
-    return bla_find(t, az, w, idx);
-}
-
-// ---------------------------------------------------------------------------
-// holomorphic models
-template <int KIND, int DEG>
-struct HoloF32 {
-    struct State { xs z, d, h, c; int32_t w, n; bool dyn, nul; };
-
-    __device__ static void init(State &s, const KArgs &a, double px, double py) {
-        // c = dx * (lin_mat . pix) + drift, assembled in binary64 mantissas
-        const double x1 = a.lin[0] * px + a.lin[1] * py;
-        const double y1 = a.lin[2] * px + a.lin[3] * py;
-        xs cp = xs_from(a.dx_m * x1, a.dx_m * y1, a.dx_e);
-        xs dr = xs_from(a.drift_m[0], a.drift_m[1], a.drift_e[0]);
-        s.c = cp + dr;
-        s.z = xs_zero(); s.d = xs_zero(); s.h = xs_zero();
-        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
-    }
-
-    __device__ static int step(State &s, const KArgs &a, Counters &k) {
-        const Tables &t = a.t;
-        const xs *Zx = (const xs *)a.tx.Zx, *DC = (const xs *)a.tx.DCx, *DZ = (const xs *)a.tx.DZx;
-        while (t.bla_on && (s.w & 7) == 0) {
-            const double az = ldexp((double)hypotf(s.z.x, s.z.y), max(s.z.e, -2000));
-            int32_t idx;
-            const int32_t st = bla_lookup_az(t, az, s.w, idx);
-            k.tries++;
-            if (st == 0) break;
-            {
-                const xs A = ((const xs *)a.tx.Ax)[idx], B = ((const xs *)a.tx.Bx)[idx];
-                s.n += st;
-                s.w += st;
-                s.z = A * s.z + B * s.c;
-                if (a.calc_dc) s.d = A * s.d;
-                if (a.calc_dz) s.h = A * s.h;
-                k.bla++; k.bla_it += st;
-            }
-        }
-        s.n += 1; k.plain++;
-        const xs Zr = Zx[s.w];
-        bool fast = false;
-        if constexpr (KIND == KIND_M2) fast = Zr.e >= -60;
-        if (fast) {
-            // z^2 + c with one shared exponent per quantity: the orbit point
-            // and z's own value (u, flushed to 0 once |z| < 2^-126, where
-            // its square is below binary32 resolution next to 2 Z z) are
-            // plain floats, so each product is 4 flops and each quantity is
-            // renormalised once per step
-            const float Zf_x = ldexpf(Zr.x, Zr.e), Zf_y = ldexpf(Zr.y, Zr.e);
-            const float ux = ldexpf(s.z.x, s.z.e), uy = ldexpf(s.z.y, s.z.e);
-            const float sx = Zf_x + ux, sy = Zf_y + uy;               // Z + z
-            float px, py;
-            if (a.calc_dc) {                                          // 2 ((Z + z) d + Zd z)
-                float qx, qy;
-                cmul(s.d.x, s.d.y, sx, sy, px, py);
-                const xs Zd = s.dyn ? xs_zero() : DC[s.w];
-                cmul(Zd.x, Zd.y, s.z.x, s.z.y, qx, qy);
-                s.d = xs_sum(px, py, s.d.e + 1, qx, qy, Zd.e + s.z.e + 1);
-            }
-            if (a.calc_dz) {                                          // 2 ((Z + z) h + Zh z)
-                float qx, qy;
-                cmul(s.h.x, s.h.y, sx, sy, px, py);
-                const xs Zh = s.nul ? DZ[0] : DZ[s.w];
-                cmul(Zh.x, Zh.y, s.z.x, s.z.y, qx, qy);
-                s.h = xs_sum(px, py, s.h.e + 1, qx, qy, Zh.e + s.z.e + 1);
-            }
-            cmul(s.z.x, s.z.y, sx + Zf_x, sy + Zf_y, px, py);         // z (z + 2 Z)
-            s.z = xs_sum(px, py, s.z.e, s.c.x, s.c.y, s.c.e);
-        } else {
-            const xt z0 = as_t(s.z), Zt = as_t(Zr);
-            if (a.calc_dc) s.d = of_t(p_deriv<KIND, DEG>(z0, as_t(s.d), Zt, as_t(s.dyn ? xs_zero() : DC[s.w])));
-            if (a.calc_dz) s.h = of_t(p_deriv<KIND, DEG>(z0, as_t(s.h), Zt, as_t(s.nul ? DZ[0] : DZ[s.w])));
-            s.z = of_t(p_zn<KIND, DEG>(z0, Zt, as_t(s.c)));
-        }
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        if (s.n >= a.max_iter) return 0;
-        if (a.calc_dz) {
-            const xs q = s.h + (s.nul ? DZ[0] : (s.n == t.order ? DZ[t.L] : DZ[s.w]));
-            if (ldexp(xs_abs2_m(q), 2 * max(q.e, -2000)) < a.eps2) return 2;
-        }
-        // escape and rebase tests in binary64
-        const cplx zd = xs_to_d(s.z);
-        const cplx Zn1 = t.Zn[s.w];
-        const cplx ZZ = zd + Zn1;
-        if (ZZ.re * ZZ.re + ZZ.im * ZZ.im > a.M2) return 1;
-        if (s.w >= t.div_iter - 1) {
-            s.z = xs_from(ZZ.re, ZZ.im, 0);
-            if (a.calc_dc) s.d = s.d + DC[s.w];
-            if (a.calc_dz) {
-                if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
-                s.nul = true;
-            }
-            s.w = 0;
-            k.reb++;
-            return -1;
-        }
-        s.dyn = (fabs(ZZ.re) <= fabs(zd.re)) && (fabs(ZZ.im) <= fabs(zd.im));
-        if (s.dyn) {
-            const xs ZZx = s.z + Zx[s.w];
-            if (xs_abs2_le(ZZx, s.z)) {
-                s.z = ZZx;
-                if (a.calc_dc) s.d = s.d + DC[s.w];
-                if (a.calc_dz) {
-                    if (!s.nul) s.h = s.h + (s.n == t.order ? DZ[t.L] : DZ[s.w]);
-                    s.nul = true;
-                }
-                s.w = 0;
-                k.reb++;
-            }
-        }
-        return -1;
-    }
-
-    __device__ static void finish(const State &s, const KArgs &a, int64_t i, int reason) {
-        const int64_t n = a.ld;
-        const cplx z = xs_to_d(s.z) + a.t.Zn[s.w];
-        a.Z[2 * i] = z.re; a.Z[2 * i + 1] = z.im;
-        int f = 1;
-        if (a.calc_dz) {
-            const cplx h = xs_to_d(s.h);
-            a.Z[2 * (n + i)] = h.re; a.Z[2 * (n + i) + 1] = h.im;
-            f = 2;
-        }
-        if (a.calc_dc) {
-            const cplx d = xs_to_d(s.d + ((const xs *)a.tx.DCx)[s.w]);
-            a.Z[2 * (f * n + i)] = d.re; a.Z[2 * (f * n + i) + 1] = d.im;
-        }
-        a.U[i] = s.w; a.stop_iter[i] = s.n; a.stop_reason[i] = (int8_t)reason;
-    }
-};
-
 // ---------------------------------------------------------------------------
 // Holomorphic models on scaled binary32 mantissas, with a running error
 // bound.
@@ -316,6 +181,15 @@ constexpr float KE = 4.f * 0x1p-24f;          // fresh error per step, x (term m
 constexpr double ERR_SIGMAS = 8.;              // error bound, in standard deviations
 constexpr float RENORM_AT = 0x1p32f;           // mantissa 1-norm that forces a renormalisation
 constexpr double INTERIOR_MARGIN = 1e-3;       // relative margin of the stationarity test
+// Steps (plain steps, each with the BLA jumps before it) after which a pixel
+// still iterating stops as REASON_UNSURE and is left to the binary64 pass.
+// The last pixels of a launch run alone in their waves, where a step's time
+// is its latency, and the binary32 step, with its error estimate and scaled
+// exponents, has the longer one: a pixel of 5e6 plain steps took 15 s in
+// the binary32 pass against 2.5 s in the binary64 one.  Below the cap the
+// pass loses nothing; above it a pixel costs at most the cap's binary32
+// steps more than in the binary64 loop alone.
+constexpr int32_t FP32X_STEP_CAP = 1 << 16;
 
 // BLA stage selection of bla_find with its certainty: |z| is known to
 // within eb, so the binary64 loop takes the same stage unless a radius lies
@@ -350,7 +224,7 @@ __device__ __forceinline__ float l2(float x, float y) { return __builtin_sqrtf(x
 
 template <int KIND, int DEG>
 struct HoloS32 {
-    struct State { xs z, c, d, h; float cx, cy, e2; int32_t w, n; bool dyn, nul; };
+    struct State { xs z, c, d, h; float cx, cy, e2; int32_t w, n, steps; bool dyn, nul; };
 
     // c in z's scale, after z's exponent moved
     __device__ static void sync_c(State &s) {
@@ -401,7 +275,7 @@ struct HoloS32 {
         s.d = xs_zero(); s.h = xs_zero();
         s.e2 = 0.f;
         sync_c(s);
-        s.w = 0; s.n = 0; s.dyn = true; s.nul = false;
+        s.w = 0; s.n = 0; s.steps = 0; s.dyn = true; s.nul = false;
     }
 
     // one plain step on mantissas: the orbit point Zf (absolute, binary32),
@@ -501,6 +375,7 @@ struct HoloS32 {
     __device__ static int step(State &s, const KArgs &a, Counters &k) {
         const Tables &t = a.t;
         const xs *Zx = (const xs *)a.tx.Zx, *DC = (const xs *)a.tx.DCx, *DZ = (const xs *)a.tx.DZx;
+        if (++s.steps > FP32X_STEP_CAP) return REASON_UNSURE;
         if ((s.w & 7) == 0) {
             renorm(s);
             while (t.bla_on && (s.w & 7) == 0) {

@@ -447,24 +447,31 @@ struct BsStd {
             }
         }
         s.n += 1; k.plain++;
+        // the step's table reads first, its tests together (as in HoloStd)
+        const int32_t w1 = (s.w + 1 >= t.order) ? 0 : s.w + 1;
         const cplx R = t.Zn[s.w];
+        const cplx R1 = t.Zn[w1];
         if (a.calc_dc) {
             double Rd[4] = {0., 0., 0., 0.};
             if (!s.dyn) { Rd[0] = t.bs[0][s.w]; Rd[1] = t.bs[1][s.w]; Rd[2] = t.bs[2][s.w]; Rd[3] = t.bs[3][s.w]; }
             bs_hess<FL>(s.x, s.y, s.P, R.re, R.im, Rd);
         }
         bs_zn<FL>(s.x, s.y, R.re, R.im, s.a, s.b);
+        const double XX = s.x + R1.re, YY = s.y + R1.im;
+        const bool t_esc = XX * XX + YY * YY > a.M2;
+        const bool reb_div = w1 >= t.div_iter - 1;
+        const bool dyn = (fabs(XX) <= fabs(s.x)) && (fabs(YY) <= fabs(s.y));
+        if constexpr (!ORB) {
+            if (!(s.n >= a.max_iter || t_esc || reb_div || dyn)) { s.w = w1; s.dyn = false; return -1; }
+        }
         if (s.n >= a.max_iter) return 0;
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        const cplx R1 = t.Zn[s.w];
-        double XX = s.x + R1.re, YY = s.y + R1.im;
+        s.w = w1;
         s.orb.record(s.n, a.backshift, XX, YY);
-        if (XX * XX + YY * YY > a.M2) return 1;
-        bool reb = (s.w >= t.div_iter - 1);
+        if (t_esc) return 1;
+        bool reb = reb_div;
         if (!reb) {
-            s.dyn = (fabs(XX) <= fabs(s.x)) && (fabs(YY) <= fabs(s.y));
-            reb = s.dyn;
+            s.dyn = dyn;
+            reb = dyn;
         }
         if (reb) {
             s.x = XX; s.y = YY;
@@ -496,13 +503,12 @@ struct BsXr {
     // orbit point i, normalised per coordinate: the register's Xrange
     // values where the binary64 orbit underflows, the binary64 values
     // elsewhere, scaled by the exponents the tables hold for them
-    __device__ static void ZR(const Tables &t, int32_t i, xlf &X, xlf &Y) {
-        const int32_t kx = t.zn_k[2 * i], ky = t.zn_k[2 * i + 1];
+    // (from the point's codes kx, ky and binary64 value R, loaded beforehand)
+    __device__ static void ZRv(const Tables &t, int32_t kx, int32_t ky, cplx R, xlf &X, xlf &Y) {
         if (kx < ZK_REGISTER) {
             const int32_t j = kx - INT32_MIN;
             X = xlf_of(t.xr_m[2 * j], t.xr_e[2 * j]); Y = xlf_of(t.xr_m[2 * j + 1], t.xr_e[2 * j + 1]);
         } else {
-            const cplx R = t.Zn[i];
             X.m = ldexp(R.re, -kx); X.e = kx;
             Y.m = ldexp(R.im, -ky); Y.e = ky;
         }
@@ -545,8 +551,13 @@ struct BsXr {
             }
         }
         s.n += 1; k.plain++;
+        // the step's table reads first, its tests together (as in HoloStd)
+        const int32_t w1 = (s.w + 1 >= t.order) ? 0 : s.w + 1;
+        const int32_t kx0 = t.zn_k[2 * s.w], ky0 = t.zn_k[2 * s.w + 1];
+        const int32_t kx1 = t.zn_k[2 * w1], ky1 = t.zn_k[2 * w1 + 1];
+        const cplx R0 = t.Zn[s.w], R1 = t.Zn[w1];
         xlf X, Y;
-        ZR(t, s.w, X, Y);
+        ZRv(t, kx0, ky0, R0, X, Y);
         // one iteration on lazily normalised values, renormalised once
         const xzf x = as_zf(s.x), y = as_zf(s.y), Xz = as_zf(X), Yz = as_zf(Y);
         if (a.calc_dc) {
@@ -563,14 +574,18 @@ struct BsXr {
         bs_zn<FL>(nx, ny, Xz, Yz, as_zf(s.a), as_zf(s.b));
         s.x = xlf_of(nx); s.y = xlf_of(ny);
         s.xs = to_std(s.x); s.ys = to_std(s.y);
+        const double XX = s.xs + R1.re, YY = s.ys + R1.im;
+        const bool t_esc = XX * XX + YY * YY > a.M2;
+        const bool reb_div = w1 >= t.div_iter - 1;
+        const bool dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
+        if constexpr (!ORB) {
+            if (!(s.n >= a.max_iter || t_esc || reb_div || dyn)) { s.w = w1; s.dyn = false; return -1; }
+        }
         if (s.n >= a.max_iter) return 0;
-        s.w += 1;
-        if (s.w >= t.order) s.w = 0;
-        const cplx R1 = t.Zn[s.w];
-        double XX = s.xs + R1.re, YY = s.ys + R1.im;
+        s.w = w1;
         s.orb.record(s.n, a.backshift, XX, YY);
-        if (XX * XX + YY * YY > a.M2) return 1;
-        if (s.w >= t.div_iter - 1) {
+        if (t_esc) return 1;
+        if (reb_div) {
             s.xs = XX; s.ys = YY;
             s.x = xlf_of(XX, 0); s.y = xlf_of(YY, 0);
             if (a.calc_dc) for (int q = 0; q < 4; q++) s.P[q] = s.P[q] + PATH(t, q, s.w);
@@ -578,10 +593,10 @@ struct BsXr {
             k.reb++;
             return -1;
         }
-        s.dyn = (fabs(XX) <= fabs(s.xs)) && (fabs(YY) <= fabs(s.ys));
+        s.dyn = dyn;
         if (s.dyn) {
             xlf X1, Y1;
-            ZR(t, s.w, X1, Y1);
+            ZRv(t, kx1, ky1, R1, X1, Y1);
             const xlf XXx = s.x + X1, YYx = s.y + Y1;
             if (le(XXx * XXx + YYx * YYx, s.x * s.x + s.y * s.y)) {
                 s.x = XXx; s.y = YYx;
