@@ -137,6 +137,19 @@ xf deriv_scale(const fsg_ref_desc *d) {
     return d->deriv_mant != 0. ? mkf(d->deriv_mant, d->deriv_exp) : mkf(d->dx_mant, d->dx_exp);
 }
 
+// Entries of the Xrange path tables whose mantissa is zero carry the zero
+// exponent code XL_ZERO_E, so that the kernels load a path value as it is,
+// without a zero test per load (the value and its scaling are unchanged: a
+// zero mantissa is zero at any exponent)
+void zero_codes(const std::vector<cplx> &m, std::vector<int32_t> &e) {
+    for (size_t k = 0; k < m.size(); k++)
+        if (m[k].re == 0. && m[k].im == 0.) e[k] = XL_ZERO_E;
+}
+void zero_codes(const std::vector<double> &m, std::vector<int32_t> &e) {
+    for (size_t k = 0; k < m.size(); k++)
+        if (m[k] == 0.) e[k] = XL_ZERO_E;
+}
+
 // dZndc: D_0 = 0, D_i = f'(Z_{i-1}) D_{i-1} + s, wrapped value at [0] for cycles
 void holo_dzndc(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e) {
     const fsg_ref_desc *d = h.d;
@@ -154,6 +167,7 @@ void holo_dzndc(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e)
         int64_t i = valid - 1;
         if (i == (int64_t)d->ref_order - 1) D[0] = D[i] * dfdz_rt(d->kind, d->degree, h.Z(i)) + s;
         for (int64_t k = 0; k < L; k++) { m[k] = mk(D[k].re, D[k].im); e[k] = D[k].e; }
+        zero_codes(m, e);
     }
 }
 
@@ -174,6 +188,7 @@ void holo_dzndz(const HostRef &h, std::vector<cplx> &m, std::vector<int32_t> &e)
         int64_t i = valid - 1;
         D[L] = dfdz_rt(d->kind, d->degree, h.Zxr(i)) * D[i];
         for (int64_t k = 0; k <= L; k++) { m[k] = mk(D[k].re, D[k].im); e[k] = D[k].e; }
+        zero_codes(m, e);
     }
 }
 
@@ -214,8 +229,10 @@ void bs_paths(const HostRef &h, std::vector<double> P[4], std::vector<int32_t> E
     };
     for (int64_t i = 1; i < valid; i++) incr_xr(i - 1, i);
     if (valid - 1 == (int64_t)d->ref_order - 1) incr_xr(valid - 1, 0);
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < 4; q++) {
         for (int64_t k = 0; k < L; k++) { P[q][k] = X[q][k].m; E[q][k] = X[q][k].e; }
+        zero_codes(P[q], E[q]);
+    }
 }
 
 int stages_of(int64_t L) {

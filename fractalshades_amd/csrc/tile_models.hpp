@@ -277,9 +277,10 @@ struct HoloXr {
     struct State { xl z, d, h, c; cplx zs; int32_t w, n; bool dyn, nul; OrbitRec<ORB> orb; };
 
     // derivative path values, unnormalised: the host's Xrange records keep
-    // |mantissa| within 2^+-100
-    __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; return xz_ld(m.re, m.im, t.dc_e[i]); }
-    __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; return xz_ld(m.re, m.im, t.dz_e[i]); }
+    // |mantissa| within 2^+-100, and a zero entry carries the zero exponent
+    // code (the tables are built so), so a value is used as it is loaded
+    __device__ static xz DCz(const Tables &t, int32_t i) { const cplx m = t.dc[i]; xz r; r.re = m.re; r.im = m.im; r.e = t.dc_e[i]; return r; }
+    __device__ static xz DZz(const Tables &t, int32_t i) { const cplx m = t.dz[i]; xz r; r.re = m.re; r.im = m.im; r.e = t.dz_e[i]; return r; }
     // an orbit point, normalised, from its code k and binary64 value z: the
     // register's Xrange value where the binary64 orbit underflows, z
     // elsewhere, scaled by the exponent k (xl_of of the point, without its
@@ -563,9 +564,10 @@ struct BsXr {
         if (a.calc_dc) {
             const xzf z0 = as_zf(xlf_zero());
             xzf Rd[4] = {z0, z0, z0, z0};
-            // the host's Xrange path records keep |mantissa| within 2^+-100: used
-            // unnormalised, as HoloXr's DCz
-            if (!s.dyn) for (int q = 0; q < 4; q++) Rd[q] = zf_mk(t.bs[q][s.w], t.bs_e[q][s.w]);
+            // the host's Xrange path records keep |mantissa| within 2^+-100 (a
+            // zero entry with the zero exponent code): used as loaded, as
+            // HoloXr's DCz
+            if (!s.dyn) for (int q = 0; q < 4; q++) { Rd[q].m = t.bs[q][s.w]; Rd[q].e = t.bs_e[q][s.w]; }
             xzf P[4] = {as_zf(s.P[0]), as_zf(s.P[1]), as_zf(s.P[2]), as_zf(s.P[3])};
             bs_hess<FL>(x, y, P, Xz, Yz, Rd);
             for (int q = 0; q < 4; q++) s.P[q] = xlf_of(P[q]);

@@ -161,14 +161,6 @@ FSG_DEV xz xz_mk(double re, double im, int32_t e) {
     r.e = (fmax(fabs(re), fabs(im)) == 0.) ? XL_ZERO_E : e;
     return r;
 }
-// the same for a value read from a table: two comparisons instead of the
-// maximum, which a loaded value would first have to canonicalise
-FSG_DEV xz xz_ld(double re, double im, int32_t e) {
-    xz r;
-    r.re = re; r.im = im;
-    r.e = (re == 0. && im == 0.) ? XL_ZERO_E : e;
-    return r;
-}
 FSG_DEV xz as_z(xl a) { xz r; r.re = a.re; r.im = a.im; r.e = a.e; return r; }
 FSG_DEV xz xz_zero() { xz r; r.re = 0.; r.im = 0.; r.e = XL_ZERO_E; return r; }
 FSG_DEV xl xl_of(xz a) { return xl_n(a.re, a.im, a.e); }

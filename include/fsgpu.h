@@ -225,7 +225,8 @@ int fsg_work_stats(const void *d_work, fsg_stats *stats);
  *   10..13: their exponents (L int32)
  * `bytes` is the size of `out`; returns FSG_E_INVALID if too small.  The
  * tables are the reference's M_bla / r_bla / dZndc_path / dZndz_path
- * (perturbation.py:1881, 2336, 2516).  fsg_ctx_bla_shape gives the BLA size.
+ * (perturbation.py:1881, 2336, 2516); an exponent whose mantissa is zero
+ * reads -2^26 (the kernels' zero code).  fsg_ctx_bla_shape gives the BLA size.
  */
 int fsg_ctx_read_table(fsg_ctx *ctx, int which, void *out, size_t bytes);
 int fsg_ctx_bla_shape(fsg_ctx *ctx, int64_t *bla_len, int32_t *stages_bla);
