@@ -326,7 +326,8 @@ struct HoloXr {
         // Every table value the step reads is loaded here, as in HoloStd: the
         // orbit point, its code and the path values at w, the next orbit
         // point and its code (escape and rebase tests) and the dZndz value of
-        // the interior test
+        // the interior test (z^d + c: after the step, whose partial products
+        // need the registers; loaded first, it spilled at d = 3)
         const int32_t w1 = (s.w + 1 >= t.order) ? 0 : s.w + 1;
         const int32_t k0 = t.zn_k[s.w], k1 = t.zn_k[w1];
         const cplx Z0 = t.Zn[s.w], Z1 = t.Zn[w1];
@@ -334,7 +335,7 @@ struct HoloXr {
         if (DC_ON(a)) { const xz v = DCz(t, s.w); Zd = s.dyn ? xz_zero() : v; }
         if (DZ_ON(a)) {
             Zh = DZz(t, s.nul ? 0 : s.w);
-            H1 = DZz(t, s.nul ? 0 : (s.n == t.order ? t.L : w1));
+            if constexpr (KIND == KIND_M2) H1 = DZz(t, s.nul ? 0 : (s.n == t.order ? t.L : w1));
         }
         const xz Zr = as_z(ZRv(t, k0, Z0));
         const xz z0 = as_z(s.z);
@@ -349,6 +350,9 @@ struct HoloXr {
             // the last sums normalised at once (add_n: no zero test of their own)
             s.z = mn_step<DEG>(z0, Zr, as_z(s.c), DC_ON(a), DZ_ON(a), s.d, s.h, as_z(s.d), as_z(s.h),
                                Zd, Zh, [](xz u, xz v) { return add_n(u, v); });
+        }
+        if constexpr (KIND == KIND_MN) {
+            if (DZ_ON(a)) H1 = DZz(t, s.nul ? 0 : (s.n == t.order ? t.L : w1));
         }
         s.zs = to_std(s.z);
         s.w = w1;
