@@ -192,7 +192,7 @@ def serving_leg(n_requests, precision, rate_frac=0.7, seed=0, layers=False, exte
     return out
 
 
-def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
+def movie_leg(n_frames, nx=480, ny=270, max_iter=500000, chunk_px=1 << 21):
     """A zoom sequence through the serving loop: n_frames frames, width
     geometric from 4 down to 1e-2000, onto one 2100-digit nucleus whose orbit
     is computed unwrapped to max_iter and kept resident (one base context per
@@ -211,6 +211,7 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
                        "because each frame's derivative paths over the whole orbit are prepared on the host",
            "frames": n_frames, "shape": f"{nx}x{ny}", "max_iter": max_iter,
            "widths": f"{specs[0].width} .. {specs[-1].width}", "orbit_points": max_iter + 1,
+           "chunk_px": chunk_px,
            "orbit_host_s": round(orbit_s, 1)}
     results = {}
     for precision in ("fp64", "fp32x"):
@@ -228,7 +229,7 @@ def movie_leg(n_frames, nx=480, ny=270, max_iter=500000):
             reqs.append((cia, c_pix))
         prep_s = time.perf_counter() - t0
         tiles = serving.layer_tiles(reqs)
-        srv = serving.TileServer(lanes=3, chunk_px=65536, hbm_budget=8 << 30)
+        srv = serving.TileServer(lanes=3, chunk_px=chunk_px, hbm_budget=8 << 30)
         try:
             rs = serving.run_stream(srv, reqs, None, tiles=tiles, posts=posts,
                                     widths=[sp.nx for sp in specs])
@@ -339,6 +340,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32x", action="store_true", help="skip the binary32 legs")
     ap.add_argument("--serve", type=int, default=512, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--movie-chunk", type=int, default=1 << 21,
+                    help="pixels per chunk of the zoom sequence's server (default: a 1920x1080 "
+                         "frame is one chunk, so its longest pixels run in one launch together)")
     ap.add_argument("--movie", type=int, default=24,
                     help="frames of the zoom-sequence leg at 1920x1080, max_iter 5e6, 5e6-point orbit (0: skip)")
     ap.add_argument("--order", choices=("grid", "linear"), default="grid",
@@ -445,7 +449,8 @@ def main():
         if lat512:
             out["tile_latency_512sq_ms"] = {"p50": lat512["p50_ms"], "p99": lat512["p99_ms"]}
     if args.movie > 0:
-        out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=1920, ny=1080, max_iter=5_000_000)
+        out["config"]["zoom_sequence"] = movie_leg(args.movie, nx=1920, ny=1080, max_iter=5_000_000,
+                                                   chunk_px=args.movie_chunk)
     if not args.no_cpu:
         base, stride, ref = cpu_baseline(leg.fields, leg.cia.arrays, leg.c_pix)
         # the same sample checked against the GPU's results of the timed frame
