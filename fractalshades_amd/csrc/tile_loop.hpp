@@ -101,8 +101,12 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // every trip on which any lane switches -- about one switch per trip when
 // each lane switches on its own.  Batching the switches trades idle lanes for
 // that cost; measured on the 2048x2048 1e-300 frame: batch 8 -> 2.51 ms,
-// 16 -> 2.19, 24 -> 2.09, 32 -> 2.02, 40 -> 1.96, 48 -> 1.96.
-constexpr int REFILL_BATCH = 40;
+// 16 -> 2.19, 24 -> 2.09, 32 -> 2.02, 40 -> 1.96, 48 -> 1.96.  Re-measured
+// once the refill's atomics were spread over the XCDs and the step's tests
+// taken together, against 40 (1e-300 frame / interior-dominated frame at
+// 5e-601 / z^3 + c at 1e-800): 24: +1 / +15 / +5 %, 32: +1 / +7 / +2 %,
+// 48: -2 / -5 / -1 %, 56: -2 / -9 / -2.5 %, 60: -3 / -10 / -2 %.
+constexpr int REFILL_BATCH = 60;
 
 template <class Model>
 __device__ __forceinline__ void tile_loop(const KArgs &a) {

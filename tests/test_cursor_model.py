@@ -17,7 +17,7 @@ def group_shift(npts):
     return min(14, max(6, max(npts >> 3, 1).bit_length() - 1))
 
 
-def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=40):
+def run(npts, nblocks, waves_per_block, seed, unsure=None, refill_batch=60):
     rnd = random.Random(seed)
     cursors = [0] * 8
     gsh = group_shift(npts)
