@@ -31,7 +31,7 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters
 # on a plain 256 MiB copy (`bench.py --calib-copy`): FETCH x2.0, WRITE x1.0.
 # Summary and commands: profiles/k_tile_C2_2048sq_1e-300.md.
-MEASURED_TRAFFIC = {("C2", 0): 317.7e6}
+MEASURED_TRAFFIC = {("C2", 0): 316.3e6}
 
 
 def calib_copy():

@@ -59,6 +59,37 @@ def test_fp32x_workloads(name):
     assert rep["escape_count_mismatch_frac"] <= 1e-4
 
 
+def test_fp32x_step_cap_hands_long_pixels_to_binary64():
+    """A pixel still iterating after the binary32 pass's step cap (65,536
+    steps) is left to the binary64 pass, which writes the drop-in's result
+    bit for bit.  The frame: width 4 around the critical orbit with BLA off
+    (the zoom sequence's shallow regime), 192x108 pixels, max_iter 300,000;
+    one pixel runs to max_iter (the oracle finds the same)."""
+    from fractalshades_amd import serving
+    nuc, _ = serving._cached_nuclei(2100)[0]
+    centre = serving.Centre(nuc, None, 2100, {})
+    sp = serving.RequestSpec(0, 192, 108, "4.000000e0", 300000, 0., 0.)
+    _, args, c_pix = serving.build_request(sp, [centre], "fp64")
+    assert args.fields["bla_on"] == 0
+    print(f"case fp32x step cap npts={c_pix.size}", flush=True)
+    ref, _ = run_gpu(args, c_pix)
+    got, stats = run_gpu(args.with_fields(precision=FP32X), c_pix)
+    long_px = ref[3].reshape(-1) > 65536
+    assert long_px.sum() >= 1
+    assert stats["redo_pixels"] >= long_px.sum()
+    for g, r in zip(got, ref):
+        gi = g.view(np.uint64) if g.dtype.kind in "fc" else g
+        ri = r.view(np.uint64) if r.dtype.kind in "fc" else r
+        gi = gi.reshape(gi.shape[0], -1)
+        ri = ri.reshape(ri.shape[0], -1)
+        cols = np.nonzero(long_px)[0]
+        if g.dtype.kind == "c":
+            cols = np.concatenate([2 * cols, 2 * cols + 1])
+        np.testing.assert_array_equal(gi[:, cols], ri[:, cols])
+    rep = error_report(got, ref, holomorphic=True)
+    assert rep["unfinished"] == 0 and rep["escape_count_mismatch_frac"] <= 1e-4
+
+
 # measured escape-count mismatch x ~1.5 (burning ship: no binary64 pass)
 DEEP_BOUND = {"C3_d3": 1e-4, "C3_d4": 1e-4, "C3_d6": 1e-4, "C3_bs": 4e-4}
 
