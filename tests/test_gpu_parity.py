@@ -52,6 +52,24 @@ def test_m2_xr_derivative_combinations_bitexact(dc, dz):
     assert hist[0] > 0 and hist[1] > 0 and (hist[2] > 0) == bool(dz)
 
 
+@pytest.mark.parametrize("dc,dz", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_m2_std_derivative_combinations_bitexact(dc, dz):
+    # the binary64 z^2 + c kernel with its tables in LDS is compiled once per
+    # combination of derivatives too: each one against the oracle on the
+    # reference's E11 frame in binary64 deltas with BLA on, where pixels stop
+    # on all three tests (max_iter, escape, interior)
+    f = frames.make("M2_E11", nx=64)
+    args, c_pix = frames.frame_inputs(f)
+    args = with_fields(args, xr_mode=0, calc_dzndc=dc, interior_detect=dz)
+    print(f"case M2_E11 std dc={dc} dz={dz} npts={c_pix.size}", flush=True)
+    ref = orc.OracleCtx(args.fields, args.arrays).cycles(c_pix)
+    got, stats = run_gpu(args, c_pix)
+    assert_same(got, ref)
+    assert stats["plain_steps"] > 0 and stats["bla_steps"] > 0
+    hist = np.bincount(ref[2].ravel().astype(np.int64), minlength=3)
+    assert hist[0] > 0 and hist[1] > 0 and (hist[2] > 0) == bool(dz)
+
+
 @pytest.mark.parametrize("bla", [0, 1])
 def test_m2_bla_switch_bitexact(bla):
     f = frames.make("M2_E20", nx=64)
