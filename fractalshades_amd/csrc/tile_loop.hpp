@@ -105,7 +105,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // once the refill's atomics were spread over the XCDs and the step's tests
 // taken together, against 40 (1e-300 frame / interior-dominated frame at
 // 5e-601 / z^3 + c at 1e-800): 24: +1 / +15 / +5 %, 32: +1 / +7 / +2 %,
-// 48: -2 / -5 / -1 %, 56: -2 / -9 / -2.5 %, 60: -3 / -10 / -2 %.
+// 48: -2 / -5 / -1 %, 56: -2 / -9 / -2.5 %, 60: -3 / -10 / -2 %; 62 and 64
+// (a refill only once every lane is idle) within 1 % of 60.
 constexpr int REFILL_BATCH = 60;
 
 template <class Model>
