@@ -74,6 +74,14 @@ __device__ __forceinline__ int32_t bla_find(const Tables &t, double az, int32_t 
     const int32_t s_inv = 31 - __builtin_clz((uint32_t)(invalid_step - 1));   // 2^s < invalid_step
     int32_t hi = min(min(3 + tz, t.stages_bla - 1), s_inv);
     int32_t lo = 3, best = 0;
+    if (hi >= 3) {
+        // the lowest stage first: invalid there, invalid at every stage (the
+        // prefix property), which settles 40 % of the lookups of the 1e-300
+        // frame in one read instead of about four (C2 -2.8 %)
+        const double r3 = t.r[2 * it];
+        if (!(az < (SCALED ? ldexp(r3, -ez) : r3))) return 0;
+        best = 3; lo = 4;
+    }
     while (lo <= hi) {
         const int32_t mid = (lo + hi) >> 1;
         const double r = t.r[2 * it + (1 << (mid - 3)) - 1];
